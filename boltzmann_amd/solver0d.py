@@ -1,0 +1,116 @@
+"""Batched 0D steady-state solver on one grid (device-resident operators + fused Newton kernel).
+
+``Batched0D`` assembles, on the GPU, the per-species collision operators (tgK = 0),
+the gas-temperature operator, the field operator and the mass matrix of one
+velocity grid (bte_0d3v_batched.assemble_operators, bte_0d3v_batched.py:310-479,
+without the M^-1 premultiplication and the Q basis, which the kernel does not
+need), builds the solver tables, and solves any batch of instances with one
+launch of bz_steady_solve_0d followed by bz_qoi_0d.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .plan0d import SolverTables
+from .vspace import C_GAMMA, QE, ME, TEMP_EV_FAC, VGrid, XSection
+
+
+class Batched0D:
+    def __init__(self, vgrid: VGrid, xsecs, growth="collisions"):
+        """growth: 'collisions' (batched path: W_s = u M^-1 C_s, bte_0d3v_batched.py:636) or
+        'all' (single path: W = u M^-1 (C + E A), bte_0d3v_solver.py:787)."""
+        self.g = vgrid
+        self.xsecs = list(xsecs)
+        species = []
+        for x in self.xsecs:
+            if x.species not in species:
+                species.append(x.species)
+        self.species = species
+        self.S = len(species)
+        g = vgrid
+        n = g.n
+        dev = g.dev
+        C = [torch.zeros((n, n), dtype=torch.float64, device=dev) for _ in species]
+        Cg = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        for x in self.xsecs:
+            C[species.index(x.species)] += g.collision_matrix(x, tgK=0.0)
+            if x.ptype == 0:
+                Cg += g.gas_temperature_matrix(x)
+        self.elastic_species = [species.index(x.species) for x in self.xsecs if x.ptype == 0]
+        self.mass_radial = g.mass_radial()
+        M = g.mass_matrix()
+        A = g.advection_matrix(scale=(1.0 / g.vth) * (QE / ME))
+        self.C, self.Cg, self.A, self.M = C, Cg, A, M
+        # QoI vectors (original order)
+        self.mass = g.mass_op()
+        self.temp = g.temp_op() * TEMP_EV_FAC
+        self.mobility = g.mobility_op()
+        self.diffusion = g.diffusion_op(self.xsecs)
+        self.rates = torch.stack([g.rate_op(x) for x in self.xsecs]).contiguous()
+        self.mw0 = 1.0 / (math.sqrt(math.pi) * g.vth) ** 3
+        self.u = (self.mass * (self.mw0 * g.vth ** 3)).contiguous()
+        gm = [True] * self.S + ([False, False] if growth == "collisions" else [True, True])
+        self.tables = SolverTables(g, C + [Cg, A, M], gm, self.u, self.mass_radial)
+        self.h0 = g.maxwellian_init()
+        self._scratch = {}
+
+    def initial(self, B):
+        return self.h0.reshape(-1, 1).repeat(1, B).contiguous()
+
+    def _get_scratch(self, B):
+        if B not in self._scratch:
+            t = self.tables
+            dev = self.g.dev
+            self._scratch = {B: dict(
+                kprof=torch.empty((B, t.P), dtype=torch.float64, device=dev),
+                urows=torch.empty((B, t.n, t.WP), dtype=torch.float64, device=dev),
+                vec=torch.empty((B, 4, t.n), dtype=torch.float64, device=dev),
+                iters=torch.empty(B, dtype=torch.int32, device=dev),
+                status=torch.empty(B, dtype=torch.int32, device=dev),
+                res_abs=torch.empty(B, dtype=torch.float64, device=dev),
+                res0=torch.empty(B, dtype=torch.float64, device=dev))}
+        return self._scratch[B]
+
+    def solve_raw(self, x, n0, Tg, E, ns_by_n0, atol=1e-10, rtol=1e-10, max_iter=30, alpha_min=1e-16,
+                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None):
+        """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0)."""
+        B = x.shape[1]
+        assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
+        for v in (n0, Tg, E):
+            assert v.shape == (B,) and v.dtype == torch.float64 and v.is_contiguous()
+        assert ns_by_n0.shape == (self.S, B) and ns_by_n0.is_contiguous()
+        sc = self._get_scratch(B)
+        bt = _lib.Batch0D(B=B, n0=n0.data_ptr(), Tg=Tg.data_ptr(), E=E.data_ptr(), ns_by_n0=ns_by_n0.data_ptr(),
+                          x=x.data_ptr(), kprof=sc["kprof"].data_ptr(), urows=sc["urows"].data_ptr(),
+                          vec=sc["vec"].data_ptr(), iters=sc["iters"].data_ptr(), status=sc["status"].data_ptr(),
+                          res_abs=sc["res_abs"].data_ptr(), res0=sc["res0"].data_ptr(), atol=float(atol),
+                          rtol=float(rtol), alpha_min=float(alpha_min), alpha_rho=float(alpha_rho),
+                          ls_c=float(ls_c), max_iter=int(max_iter), mode=int(mode), tg_scale=float(tg_scale))
+        self._last_batch = bt
+        _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
+        return sc
+
+    def qois(self, h, n0, E, effective_mobility=True, stream=None):
+        """Normalised distribution and QoIs (bte_0d3v_batched.py:1163-1217)."""
+        g = self.g
+        n, B = h.shape
+        G = len(self.xsecs)
+        hout = torch.empty_like(h)
+        out = torch.empty((B, 3 + G), dtype=torch.float64, device=g.dev)
+        mm_fac = (1.0 / (2.0 * math.sqrt(math.pi))) * 4.0 * math.pi
+        mm_scale = 2.0 * (g.vth / C_GAMMA) ** 3 / mm_fac
+        _lib.call("bz_qoi_0d", n, g.num_sh, B, G, h, self.mass, self.temp, self.mobility, self.diffusion,
+                  self.rates, E, n0, mm_scale, self.mw0 * g.vth ** 3, self.mw0 * g.vth ** 5, C_GAMMA,
+                  1 if effective_mobility else 0, hout, out, stream=stream)
+        q = {"energy": out[:, 0], "mobility": out[:, 1], "diffusion": out[:, 2], "rates": out[:, 3:].t()}
+        return hout, q
+
+    def solve(self, n0, Tg, E, ns_by_n0, x0=None, mode=1, **kw):
+        B = n0.shape[0]
+        x = (self.initial(B) if x0 is None else x0.clone()).contiguous()
+        sc = self.solve_raw(x, n0, Tg, E, ns_by_n0, mode=mode, **kw)
+        h, q = self.qois(x, n0, E)
+        info = {k: sc[k] for k in ("iters", "status", "res_abs", "res0")}
+        return h, q, info

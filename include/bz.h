@@ -1,0 +1,148 @@
+/* C ABI of libbz.so: MI355X (gfx950) kernels for the velocity-space hot path of the
+ * B-spline x spherical-harmonic Boltzmann solver.
+ *
+ * Conventions
+ *   - every pointer argument is a device pointer (hipMalloc / torch tensor memory)
+ *     unless stated otherwise; the caller owns every buffer;
+ *   - all arithmetic is FP64; matrices are row-major;
+ *   - every entry point is asynchronous on `stream` and returns 0 on success,
+ *     -1 (BZ_EINVAL) for invalid sizes/arguments (nothing launched),
+ *     -2 (BZ_EHIP) when a HIP launch error is reported.
+ *   - radial-major coefficient index p*ns + l (p: B-spline, l: spherical harmonic, m = 0).
+ *
+ * Each declaration names the reference interface it replaces
+ * (ut-padas/boltzmann, paths relative to BESolver/python/).
+ */
+#ifndef BZ_H
+#define BZ_H
+#include <hip/hip_runtime.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ assembly */
+
+/* out[p*npr+k] = sum_x w x^pw B_p^(da)(x) B_k^(db)(x) over the cubic B-spline knot
+ * vector t (npr+4 knots), q Gauss-Legendre nodes (glx, glw on [-1,1]) per interval.
+ * Replaces the spline branches of SpectralExpansionSpherical.compute_mass_matrix
+ * (spec_spherical.py:226-277) and compute_advection_matix (spec_spherical.py:442-498). */
+int bz_radial_bilinear(const double* t, int npr, const double* glx, const double* glw, int q, int pw, int da,
+                       int db, double* out, hipStream_t stream);
+
+/* Radial blocks of the binary electron-heavy collision operator: q0 (l = 0) and q1
+ * (l >= 1), ptype 0 elastic / 1 excitation / 2 ionization / 3 attachment, cross
+ * section given as a piecewise-linear table (tab_e [eV], tab_s [m^2]).
+ * Replaces CollisionOpSP._LOp_eulerian_radial_only, CG branch
+ * (collision_operator_spherical.py:79-225) reached through assemble_mat (:580-586). */
+int bz_collision_radial(const double* t, int npr, const double* glx, const double* glw, int q, int ptype,
+                        double vth, double c_gamma, double mByM, double threshold, double att_energy, double kappa,
+                        const double* tab_e, const double* tab_s, int tab_len, double* q0, double* q1,
+                        hipStream_t stream);
+
+/* Gas-temperature diffusion radial block (l = 0).
+ * Replaces CollisionOpSP.electron_gas_temperature (collision_operator_spherical.py:443-578). */
+int bz_gas_temperature_radial(const double* t, int npr, const double* glx, const double* glw, int q, double vth,
+                              double c_gamma, double kappa1, const double* tab_e, const double* tab_s, int tab_len,
+                              double* out, hipStream_t stream);
+
+/* out[p] = scale * sum_x w x^pw g(x) B_p(x); g = 1 (mode 0), sum of the tables (mode 1),
+ * 1/sum of the tables (mode 2), exp(-x^2) (mode 3); tables concatenated with offsets
+ * tab_off[ntab+1] (device).  Mode 3 is the Maxwellian projection of utils.function_to_basis
+ * (utils.py:475-532).
+ * Replaces utils.mass_op, temp_op, reaction_rates_op, mobility_op, diffusion_op
+ * (utils.py:73-150, 321-395, 854-944). */
+int bz_radial_moment(const double* t, int npr, const double* glx, const double* glw, int q, int pw, int mode,
+                     double scale, double vth, double c_gamma, const double* tab_e, const double* tab_s,
+                     const int* tab_off, int ntab, double* out, hipStream_t stream);
+
+/* Dense (n x n, n = npr*ns) operator from radial blocks on the (l,l) diagonal:
+ * block lmap[l] (-1 = zero) times scale.  Scatter step of the assembly routines
+ * (collision_operator_spherical.py:136-144, 214-222; spec_spherical.py:270-275). */
+int bz_scatter_diag_blocks(const double* blk, int npr, int ns, const int* lmap, double scale, double* out,
+                           hipStream_t stream);
+
+/* Dense E-field advection operator from mm1, mm2 and the l-coupling coefficients
+ * cup[2*ns] (AM, AD of l = q+1) and cdn[2*ns] (BM, BD of l = q-1).
+ * Replaces the coupling loop of compute_advection_matix (spec_spherical.py:480-498). */
+int bz_advection_scatter(const double* mm1, const double* mm2, int npr, int ns, const double* cup,
+                         const double* cdn, double scale, double* out, hipStream_t stream);
+
+/* out[k*nx + i] = B_k(x_i) (dorder 0) or B_k'(x_i) (dorder 1), zero outside the support.
+ * Replaces SpectralExpansionSpherical.Vq_r / Vdq_r (spec_spherical.py:338-383). */
+int bz_basis_table(const double* t, int npr, const double* x, int nx, int dorder, double* out, hipStream_t stream);
+
+/* Banded Cholesky (half bandwidth hb) of an npr x npr SPD block: lb[p*(hb+1)+d] = L[p,p-d].
+ * And the dense inverse of L L^T (work: npr*npr doubles).
+ * Replace SpectralExpansionSpherical.inverse_mass_mat (spec_spherical.py:299-336). */
+int bz_band_cholesky(const double* A, int npr, int hb, double* lb, hipStream_t stream);
+int bz_band_chol_inverse(const double* lb, int npr, int hb, double* inv, double* work, hipStream_t stream);
+
+/* ------------------------------------------------------------------ 0D batched solver */
+
+/* Per-grid solver tables (device pointers), vectors in the permuted order
+ * i' = (npr-1-p)*ns + l.  Built by the Python host (boltzmann_amd/plan0d.py). */
+typedef struct {
+  int n, ns, npr, bw, S, nop, P, nslots;
+  const int* first_col;   /* [n]   first column of each row profile            */
+  const int* prof_off;    /* [n+1] offsets of the row profiles                  */
+  const double* opprof;   /* [nop][P]  C_0..C_{S-1}, C_Tg, A_E, M in profiles   */
+  const int* slot_first;  /* [nslots] first row of every lane slot (-1: none)  */
+  const int* sched_next;  /* [n]   next row of the same slot (-1: none)         */
+  const double* muhat;    /* [n]   M uhat                                       */
+  const double* uhat;     /* [n]   u / |u|                                      */
+  const double* u;        /* [n]   mass vector u                                */
+  const double* wsp;      /* [nop-1][n] W_m = Op_m^T M^-1 u (zero row: operator not in the growth term) */
+  const double* mchol;    /* [n][4] banded Cholesky rows of M                   */
+} bz_grid0d_t;
+
+/* Per-batch parameters, state and scratch (device pointers). */
+typedef struct {
+  int B;
+  const double* n0;       /* [B]                                               */
+  const double* Tg;       /* [B]                                               */
+  const double* E;        /* [B]  field [V/m]                                  */
+  const double* ns_by_n0; /* [S][B]                                            */
+  double* x;              /* (n, B): initial guess in, solution out (unnormalised) */
+  double* kprof;          /* [B][P] scratch                                    */
+  double* urows;          /* [B][n][W rounded up to even] scratch              */
+  double* vec;            /* [B][4][n] scratch                                 */
+  int* iters;             /* [B]                                               */
+  int* status;            /* [B] 0 converged, 1 iteration cap, 2 line search failed */
+  double* res_abs;        /* [B] final |r|                                     */
+  double* res0;           /* [B] initial |r|                                   */
+  double atol, rtol, alpha_min, alpha_rho, ls_c;
+  int max_iter;
+  int mode;               /* 0: reference Jacobian, 1: exact Jacobian          */
+  double tg_scale;        /* C_Tg coefficient is n0 * Tg * tg_scale            */
+} bz_batch0d_t;
+
+/* Batched damped-Newton steady state of all B instances in one launch (one wavefront
+ * per instance, every instance stops on its own criterion).
+ * Replaces bte_0d3v_batched.steady_state_solve + get_rhs_and_jacobian + batched_inv
+ * (bte_0d3v_batched.py:602-821, 856-979). */
+int bz_steady_solve_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, hipStream_t stream);
+
+/* Window width (columns) the solver uses for an upper bandwidth bw (-1: unsupported). */
+int bz_solver0d_window(int bw);
+
+/* Normalisation and QoIs (energy, mobility, diffusion, G rates) of B solutions.
+ * Replaces bte_0d3v_batched.normalized_distribution + compute_QoIs
+ * (bte_0d3v_batched.py:1163-1217).  out[b*(3+G) + {0,1,2,3..}]. */
+int bz_qoi_0d(int n, int ns, int B, int G, const double* h, const double* mass, const double* temp,
+              const double* mob, const double* dif, const double* rates, const double* E, const double* n0,
+              double mm_scale, double mw0v3, double mw0v5, double c_gamma, int effective, double* hout, double* out,
+              hipStream_t stream);
+
+/* Setup helpers (no direct reference counterpart; steps inside
+ * bte_0d3v_batched.assemble_operators, bte_0d3v_batched.py:421-442). */
+int bz_gather_profile(const double* dense, int n, int ns, int npr, const int* first_col, const int* prof_off,
+                      double scale, double* out, hipStream_t stream);
+int bz_gemv(const double* A, int m, int k, int trans, const double* x, double alpha, double* y, hipStream_t stream);
+int bz_permute(const double* in, int n, int ns, int npr, int inverse, double* out, hipStream_t stream);
+int bz_mchol_perm(const double* lrev, int ns, int npr, double* out, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BZ_H */

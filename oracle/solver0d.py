@@ -1,0 +1,325 @@
+"""Oracle: 0D steady-state solvers (single and batched), QoIs (FP64, numpy).
+
+Restates (BESolver/python/...):
+  * bte_0d3v_batched.assemble_operators       bte_0d3v_batched.py:310-479 (no e-e here)
+  * get_rhs_and_jacobian (no e-e branch)      bte_0d3v_batched.py:602-646, 763-821
+  * steady_state_solve (Armijo Newton)        bte_0d3v_batched.py:882-979
+  * normalized_distribution / compute_QoIs    bte_0d3v_batched.py:1163-1217
+  * single path _QRQt, newton_solve,
+    steady_state_solver, compute_QoIs        bte_0d3v_solver.py:404-460, 24-73, 741-859, 1234-1276
+
+Pinned by tests/test_oracle_solver.py: the single-path fixed point equals the
+generalized eigenvector of (C + E A, M) with u^T h = 1 (scipy.linalg.eig);
+the batched fixed point satisfies the same eigen relation up to the
+reference's boundary-flux term (W excludes advection); E = 0 elastic-only
+gives the Maxwellian at Tg; Maxwellian closed forms pin the QoI vectors.
+"""
+import math
+import numpy as np
+
+from . import operators as op
+
+
+class BatchedOps:
+    """Per-grid operators of the batched path (bte_0d3v_batched.py:360-442)."""
+
+    def __init__(self, g: op.Grid, procs, ee=None):
+        self.g = g
+        self.procs = procs
+        species = []
+        for p in procs:
+            if p.species not in species:
+                species.append(p.species)
+        self.species = species
+        n = g.n
+        M = op.mass_matrix(g)
+        Minv = op.inverse_mass_matrix(M)
+        FOp = [np.zeros((n, n)) for _ in species]
+        FOp_g = np.zeros((n, n))
+        for p in procs:
+            FOp[species.index(p.species)] += op.collision_matrix(g, p, tgK=0.0)
+            if p.type == "ELASTIC":
+                FOp_g += op.gas_temperature_matrix(g, p)
+        self.M = M
+        self.Minv = Minv
+        self.FOp = FOp                 # un-premultiplied collision operators, per species
+        self.FOp_g = FOp_g
+        self.adv_raw = op.advection_matrix(g)
+        self.c_en = np.array([Minv @ F for F in FOp])
+        self.c_gT = Minv @ FOp_g
+        self.adv = (Minv @ self.adv_raw) * (1.0 / g.vth) * (op.QE / op.ME)
+        self.mass = op.mass_op(g)
+        self.temp = op.temp_op(g) * op.TEMP_EV_FAC
+        self.mobility = op.mobility_op(g)
+        self.diffusion = op.diffusion_op(g, procs)
+        self.rates = np.array([op.rate_op(g, p) for p in procs])
+        mw0 = op.maxwellian0(g.vth)
+        self.u = self.mass * mw0 * g.vth ** 3
+        self.Q, self.R = qr_basis(self.u, g.num_p, g.num_sh)
+        self.W = np.einsum("v,svu->su", self.u, self.c_en)
+        self.ee = ee
+
+    def initial(self, B):
+        h = op.maxwellian_init(self.g, self.Minv)
+        return np.repeat(h.reshape(-1, 1), B, axis=1)
+
+
+def qr_basis(u, num_p, num_sh):
+    """bte_0d3v_batched.py:425-442: QR of I - p p^T, drop column/row (num_p-1)*num_sh."""
+    n = u.shape[0]
+    pv = u.reshape((n, 1)) / np.sqrt(np.dot(u, u))
+    Impp = np.eye(n) - pv @ pv.T
+    Qm, Rm = np.linalg.qr(Impp)
+    k = (num_p - 1) * num_sh + 0
+    Q = np.delete(Qm, k, axis=1)
+    R = np.delete(Rm, k, axis=0)
+    assert np.linalg.norm(Impp - Q @ R) / np.linalg.norm(Impp) < 1e-10
+    assert np.linalg.norm(Q.T @ Q - np.eye(n - 1)) < 1e-10
+    return Q, R
+
+
+def batched_res_jac(ops: BatchedOps, params, jacobian="reference"):
+    """get_rhs_and_jacobian, no e-e branch (bte_0d3v_batched.py:763-821).
+
+    params: dict with n0 (B,), Tg (B,), E (B,), ns_by_n0 (S,B) [ne (B,) for e-e].
+    jacobian="exact" adds the derivative of the growth term, -Q^T x (w Q), which the
+    reference Jacobian omits; both share the residual and hence the fixed point."""
+    Q, QT = ops.Q, ops.Q.T
+    QT_Cen = np.einsum("mu,suv->smv", QT, ops.c_en)
+    QT_Cgt = QT @ ops.c_gT
+    QT_A = QT @ ops.adv
+    QT_Cen_Q = np.einsum("av,svu->sau", QT, ops.c_en @ Q)
+    QT_Cgt_Q = QT @ ops.c_gT @ Q
+    QT_A_Q = QT @ ops.adv @ Q
+    W = ops.W
+    n0, Tg, E, ns_by_n0 = params["n0"], params["Tg"], params["E"], params["ns_by_n0"]
+    I = np.eye(QT.shape[0])
+
+    def res(x):
+        return (n0 * (np.einsum("sx,svx->vx", ns_by_n0, np.einsum("svu,ux->svx", QT_Cen, x))
+                      + Tg * (QT_Cgt @ x))
+                + E * (QT_A @ x)
+                - n0 * np.einsum("sx,sx->x", ns_by_n0, W @ x) * (QT @ x))
+
+    def jac(x):
+        ns = n0 * ns_by_n0
+        L = np.einsum("sx,svu->xvu", ns, QT_Cen_Q) + np.einsum("a,bc->abc", n0 * Tg, QT_Cgt_Q)
+        mu = np.einsum("sx,sx->x", ns, W @ x)
+        J = L + np.einsum("a,bc->abc", E, QT_A_Q) - np.einsum("a,bc->abc", mu, I)
+        if jacobian == "exact":
+            wq = np.einsum("sx,su,uk->xk", ns, W, Q)
+            J = J - np.einsum("vx,xk->xvk", QT @ x, wq)
+        return J
+
+    return res, jac
+
+
+def steady_state_solve(ops: BatchedOps, params, f0, atol, rtol, max_iter,
+                       alpha_min=1e-16, alpha_rho=0.5, line_search_c=1e-4,
+                       res_jac=None, return_info=False, jacobian="reference"):
+    """bte_0d3v_batched.py:882-979 (batch-coupled loop exactly as written)."""
+    u = ops.u
+    Q, R = ops.Q, ops.R
+    res_func, jac_func = res_jac if res_jac is not None else batched_res_jac(ops, params, jacobian)
+    n_pts = f0.shape[1]
+    f0 = f0 / (u @ f0)
+    fb_prev = R @ f0
+    f1 = u / np.dot(u, u)
+    h_prev = f1[:, None] + Q @ fb_prev
+    r0 = np.linalg.norm(res_func(h_prev), axis=0)
+    abs_error = r0.copy()
+    it = 0
+    h_curr = h_prev
+    diverged = False
+    while (abs_error > atol + rtol * r0).any() and it < max_iter:
+        L = jac_func(h_prev)
+        rhs = res_func(h_prev)
+        abs_error = np.linalg.norm(rhs, axis=0)
+        Linv = np.linalg.inv(L)
+        pp_mat = np.einsum("abc,ca->ba", Linv, -rhs)
+        p = Q @ pp_mat
+        alpha = np.ones(n_pts)
+        rf_new = 0.5 * np.linalg.norm(res_func(h_prev + alpha * p), axis=0) ** 2
+        while (rf_new > 0.5 * abs_error ** 2 - line_search_c * alpha * abs_error ** 2).any():
+            rc = rf_new > 0.5 * abs_error ** 2 - line_search_c * alpha * abs_error ** 2
+            alpha[rc] *= alpha_rho
+            if (alpha < alpha_min).all():
+                diverged = True
+                break
+            rf_new = 0.5 * np.linalg.norm(res_func(h_prev + alpha * p), axis=0) ** 2
+        if diverged:
+            break
+        h_curr = h_prev + alpha * p
+        if it % 10 == 0:
+            abs_error = np.linalg.norm(res_func(h_curr), axis=0)
+        h_prev = h_curr
+        it += 1
+    final_abs = np.linalg.norm(res_func(h_curr), axis=0)
+    h = normalized_distribution(ops, h_curr)
+    qoi = compute_qois(ops, params, h)
+    if return_info:
+        return h, qoi, {"iters": it, "abs": final_abs, "rel": final_abs / r0, "diverged": diverged}
+    return h, qoi
+
+
+def steady_state_solve_each(ops: BatchedOps, params, f0, atol, rtol, max_iter, **kw):
+    """The batched loop run on every instance alone (batch of one)."""
+    B = f0.shape[1]
+    hs, qs, infos = [], [], []
+    for i in range(B):
+        pi = {k: (v[..., i:i + 1] if isinstance(v, np.ndarray) else v) for k, v in params.items()}
+        h, q, info = steady_state_solve(ops, pi, f0[:, i:i + 1], atol, rtol, max_iter,
+                                        return_info=True, **kw)
+        hs.append(h)
+        qs.append(q)
+        infos.append(info)
+    h = np.concatenate(hs, axis=1)
+    qoi = {k: np.concatenate([q[k] for q in qs], axis=-1) for k in qs[0]}
+    return h, qoi, infos
+
+
+def normalized_distribution(ops: BatchedOps, ff):
+    """bte_0d3v_batched.py:1204-1217."""
+    g = ops.g
+    mm_fac = (1.0 / (2.0 * math.sqrt(math.pi))) * 4.0 * math.pi
+    scale = (ops.mass / mm_fac) @ ff * (2.0 * (g.vth / op.C_GAMMA) ** 3)
+    return ff / scale
+
+
+def compute_qois(ops: BatchedOps, params, ff, effective_mobility=True):
+    """bte_0d3v_batched.py:1163-1202."""
+    g = ops.g
+    ns = g.num_sh
+    mw0 = op.maxwellian0(g.vth)
+    vth = g.vth
+    c = op.C_GAMMA
+    mm = (ops.mass @ ff) * mw0 * vth ** 3
+    mu = 1.5 * (ops.temp @ ff) * mw0 * vth ** 5 / mm
+    E, n0 = params["E"], params["n0"]
+    if effective_mobility:
+        M = (ops.mobility @ (math.sqrt(3) * ff[1::ns, :])) * (-(c / (3 * (E / n0))))
+    else:
+        M = (ops.mobility @ (math.sqrt(3) * ff[1::ns, :])) * (-(c / (3 * (1.0 / n0))))
+    D = (ops.diffusion @ ff[0::ns, :]) * (c / 3.0)
+    rr = ops.rates @ ff[0::ns, :]
+    return {"energy": mu, "mobility": M, "diffusion": D, "rates": rr}
+
+
+# ----------------------------------------------------------------------------- single path
+
+class SingleOps:
+    """bte_0d3v._setup_bsp_sph + _qoi_ops + _QRQt (bte_0d3v_solver.py:161-276, 372-460)."""
+
+    def __init__(self, g: op.Grid, procs, n0, Tg, ns_by_n0=None):
+        self.g = g
+        self.procs = procs
+        species = []
+        for p in procs:
+            if p.species not in species:
+                species.append(p.species)
+        if ns_by_n0 is None:
+            ns_by_n0 = np.ones(len(species))
+        self.n0 = n0
+        self.M = op.mass_matrix(g)
+        self.Minv = op.inverse_mass_matrix(self.M)
+        C = np.zeros((g.n, g.n))
+        for p in procs:
+            x = ns_by_n0[species.index(p.species)] * n0
+            if p.type == "ATTACHMENT":
+                C += x ** 2 * op.collision_matrix(g, p, tgK=Tg)
+            else:
+                C += x * op.collision_matrix(g, p, tgK=Tg)
+        self.C = C
+        self.Araw = op.advection_matrix(g)
+        self.mass = op.mass_op(g)
+        self.temp = op.temp_op(g)
+        self.mobility = op.mobility_op(g)
+        self.diffusion = op.diffusion_op(g, procs)
+        self.rates = np.array([op.rate_op(g, p) for p in procs])
+        mw0 = op.maxwellian0(g.vth)
+        self.u = self.mass * mw0 * g.vth ** 3
+        self.Q, self.R = qr_basis(self.u, g.num_p, g.num_sh)
+
+
+def newton_solve(h0, res_func, jac_func, f1, Q, R, rtol=1e-5, atol=1e-5, max_iter=1000):
+    """bte_0d3v_solver.py:24-73."""
+    abs_error = 1.0
+    rel_error = 1.0
+    it = 0
+    fb_prev = R @ h0
+    h_prev = f1 + Q @ fb_prev
+    h_curr = h_prev
+    while (rel_error > rtol and abs_error > atol) and it < max_iter:
+        L = jac_func(h_prev)
+        rhs = -res_func(h_prev)
+        abs_error = np.linalg.norm(rhs)
+        p = np.linalg.solve(L, rhs)
+        p = Q @ p
+        alpha = 1.0
+        diverged = False
+        while np.linalg.norm(res_func(h_prev + alpha * p)) > abs_error:
+            alpha *= 0.5
+            if alpha < 1e-30:
+                diverged = True
+                break
+        if diverged:
+            break
+        h_curr = h_prev + alpha * p
+        if it % 10 == 0:
+            rel_error = np.linalg.norm(h_prev - h_curr) / np.linalg.norm(h_curr)
+        h_prev = h_curr
+        it += 1
+    return h_curr, abs_error, rel_error, it
+
+
+def single_steady_state(ops: SingleOps, E_field, h_init=None, rtol=1e-13, atol=1e-8, max_iter=300):
+    """bte_0d3v_solver.py:741-859 (no e-e)."""
+    g = ops.g
+    Emat = ops.Araw * (E_field / g.vth) * (op.QE / op.ME)
+    CpE = ops.Minv @ (ops.C + Emat)
+    u, Q, R = ops.u, ops.Q, ops.R
+    QT = Q.T
+    W = u @ CpE
+
+    def res(x):
+        return QT @ (CpE @ x) - (W @ x) * (QT @ x)
+
+    def jac(x):
+        return (QT @ CpE - (W @ x) * QT) @ Q
+
+    f1 = u / np.dot(u, u)
+    if h_init is None:
+        h_init = op.maxwellian_init(g, ops.Minv)
+    h_init = h_init / np.dot(u, h_init)
+    h, a, r, it = newton_solve(h_init, res, jac, f1, Q, R, rtol=rtol, atol=atol, max_iter=max_iter)
+    sol = np.zeros((2, g.n))
+    sol[0] = normalized_single(ops, h_init)
+    sol[1] = normalized_single(ops, h)
+    return {"sol": sol, "atol": a, "rtol": r, "iters": it, "tgrid": None, "L": CpE}
+
+
+def normalized_single(ops, f):
+    """utils.py:980-997."""
+    mm_fac = (1.0 / (2.0 * math.sqrt(math.pi))) * 4.0 * math.pi
+    scale = np.dot(f, ops.mass / mm_fac) * (2 * (ops.g.vth / op.C_GAMMA) ** 3)
+    return f / scale
+
+
+def single_qois(ops: SingleOps, hh, E_field, effective_mobility=True):
+    """bte_0d3v_solver.py:1234-1276."""
+    g = ops.g
+    ns = g.num_sh
+    hh = hh.reshape((-1, g.n))
+    mw0 = op.maxwellian0(g.vth)
+    vth = g.vth
+    c = op.C_GAMMA
+    eavg_to_K = 2.0 / (3.0 * op.KB)
+    mm = (hh @ ops.mass) * mw0 * vth ** 3
+    mu = (hh @ ops.temp) * mw0 * vth ** 5 * (0.5 * op.ME * eavg_to_K / mm) * 1.5 / op.TEMP_K_1EV
+    if effective_mobility:
+        M = (math.sqrt(3) * hh[:, 1::ns]) @ ops.mobility * (-(c / (3 * (E_field / ops.n0))))
+    else:
+        M = (math.sqrt(3) * hh[:, 1::ns]) @ ops.mobility * (-(c / (3 * (1.0 / ops.n0))))
+    D = (hh[:, 0::ns] @ ops.diffusion) * (c / 3.0)
+    rr = [hh[:, 0::ns] @ ops.rates[i] for i in range(len(ops.procs))]
+    return {"energy": mu, "mobility": M, "diffusion": D, "rates": rr}

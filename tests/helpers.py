@@ -1,0 +1,68 @@
+"""Shared builders for the parity tests: the same grid/processes for oracle and product."""
+import numpy as np
+
+from conftest import XSEC_FILE
+from oracle import operators as op
+from oracle import solver0d as s0
+from oracle import xsec
+
+TE = 4.0
+EV_MAX = 80.0
+
+
+def oracle_procs():
+    recs, _ = xsec.read_cross_section_data(XSEC_FILE)
+    return op.processes_from_lxcat(recs)
+
+
+def oracle_batched(Nr, lmax, q=5, extend=True):
+    procs = oracle_procs()
+    g = op.grid_batched(Nr, lmax, TE, EV_MAX, q, [p.threshold for p in procs], extend=extend)
+    return s0.BatchedOps(g, procs), g, procs
+
+
+def instance_params(B, seed=20251202, S=3):
+    """E/N log-uniform in [1, 1000] Td, per-process density factors U[0.8, 1.2], Tg = 300 K."""
+    rng = np.random.default_rng(seed)
+    EbyN = 10.0 ** rng.uniform(0.0, 3.0, B)
+    fac = rng.uniform(0.8, 1.2, (S, B))
+    n0 = np.full(B, 3.22e22)
+    return dict(n0=n0, Tg=np.full(B, 300.0), E=EbyN * n0 * 1e-21, ns_by_n0=fac, EbyN=EbyN)
+
+
+def product_grid(g_oracle):
+    """VGrid with exactly the oracle's knot vector."""
+    from boltzmann_amd.vspace import VGrid
+    return VGrid(g_oracle.t, g_oracle.num_p, g_oracle.lmax, g_oracle.q, g_oracle.vth)
+
+
+def product_xsecs():
+    from boltzmann_amd.cross_section import xsections_from_file
+    return xsections_from_file(XSEC_FILE)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+RATE_FLOOR = 1e-10
+
+
+def qoi_errors(q_gpu, q_ref, floor=RATE_FLOOR):
+    """Max relative error per QoI; rates below floor * max_g rate are compared absolutely,
+    |dk| / max_g k (the tiny-rate rule).  The floor is 1e-10: two FP64 solvers converged
+    to roundoff already differ by ~1e-8 relative on rates near 1e-11 * max_g k."""
+    out = {}
+    for k in ("energy", "mobility", "diffusion"):
+        a = np.asarray(q_gpu[k]).reshape(-1)
+        b = np.asarray(q_ref[k]).reshape(-1)
+        out[k] = float(np.max(np.abs(a - b) / np.abs(b)))
+    ra = np.asarray(q_gpu["rates"])
+    rb = np.asarray(q_ref["rates"])
+    kmax = np.max(np.abs(rb), axis=0, keepdims=True)
+    small = np.abs(rb) < floor * kmax
+    denom = np.where(small, kmax, np.abs(rb))
+    out["rates"] = float(np.max(np.abs(ra - rb) / denom))
+    return out

@@ -1,0 +1,155 @@
+"""CPU checks of the 0D solver formulation and of the lane/slot schedule.
+
+* the bordered step in the original basis equals the reference's Q-basis Newton
+  step (bte_0d3v_batched.py:930-934) for both Jacobians;
+* a line-by-line emulation of the kernel's factorisation loop (lane slots,
+  circular windows, 2W prefetch, retire/enter) reproduces dense LU without
+  pivoting of K - mu M in the permuted order.
+"""
+import numpy as np
+import pytest
+
+from helpers import instance_params, oracle_batched
+from oracle import solver0d as s0
+from boltzmann_amd.plan0d import PREFETCH_WINDOWS, permutation, plan_structure
+
+
+def window_fn(bw):
+    for W in (8, 11, 17):
+        if bw + 1 <= W:
+            return W
+    return -1
+
+
+def dense_ops(ops, prm, i):
+    n0, Tg, E = prm["n0"][i], prm["Tg"][i], prm["E"][i]
+    ns = prm["ns_by_n0"][:, i]
+    from oracle import operators as op
+    K = sum(n0 * ns[s] * ops.FOp[s] for s in range(len(ops.FOp))) + n0 * Tg * ops.FOp_g \
+        + E * ops.adv_raw * (1.0 / ops.g.vth) * (op.QE / op.ME)
+    w = n0 * np.einsum("s,su->u", ns, ops.W)
+    return K, w
+
+
+@pytest.fixture(scope="module")
+def small():
+    ops, g, procs = oracle_batched(24, 2)
+    prm = instance_params(4, seed=3)
+    return ops, g, prm
+
+
+@pytest.mark.parametrize("jac", ["reference", "exact"])
+def test_bordered_step_equals_reference_step(small, jac):
+    ops, g, prm = small
+    res, jacf = s0.batched_res_jac(ops, prm, jac)
+    x = ops.initial(4)
+    x = x / (ops.u @ x)
+    r = res(x)
+    J = jacf(x)
+    uh = ops.u / np.linalg.norm(ops.u)
+    for i in range(4):
+        p_ref = ops.Q @ np.linalg.solve(J[i], -r[:, i])
+        K, w = dense_ops(ops, prm, i)
+        xi = x[:, i]
+        mu = w @ xi
+        A = K - mu * ops.M
+        z1 = np.linalg.solve(A, ops.M @ uh)
+        z2 = np.linalg.solve(A, ops.M @ xi)
+        if jac == "reference":
+            c, d = (uh @ xi) / (uh @ z1), 0.0
+        else:
+            S = np.array([[uh @ z1, uh @ z2], [w @ z1, w @ z2 - 1.0]])
+            c, d = np.linalg.solve(S, [uh @ xi, mu])
+        p = c * z1 + d * z2 - xi
+        assert np.linalg.norm(p - p_ref) / np.linalg.norm(p_ref) < 1e-9
+
+
+def test_residual_norm_equals_reference(small):
+    ops, g, prm = small
+    res, _ = s0.batched_res_jac(ops, prm)
+    x = ops.initial(4)
+    x = x / (ops.u @ x)
+    uh = ops.u / np.linalg.norm(ops.u)
+    r = res(x)
+    for i in range(4):
+        K, w = dense_ops(ops, prm, i)
+        R = ops.Minv @ (K @ x[:, i]) - (w @ x[:, i]) * x[:, i]
+        rp = R - (uh @ R) * uh
+        assert abs(np.linalg.norm(rp) - np.linalg.norm(r[:, i])) / np.linalg.norm(r[:, i]) < 1e-10
+
+
+def emulate_factor(A, st):
+    """Line-by-line restatement of factor_forward<W,R> (bz_solver0d.hip) on the host."""
+    n, bw, W, R = st["n"], st["bw"], st["W"], st["R"]
+    fcs, poff = st["first_col"], st["prof_off"]
+    prof = np.zeros(st["P"])
+    for i in range(n):
+        f = fcs[i]
+        prof[poff[i]:poff[i + 1]] = A[i, f:f + poff[i + 1] - poff[i]]
+    nslot = st["nslots"]
+
+    def val(row, c):
+        if row < 0:
+            return 0.0
+        f, l = fcs[row], min(row + bw, n - 1)
+        return prof[poff[row] - f + c] if f <= c <= l else 0.0
+
+    row = list(st["slot_first"])
+    w = np.zeros((nslot, W))
+    nk = np.zeros((nslot, W))
+    for s in range(nslot):
+        for k in range(W):
+            w[s, k] = val(row[s], k)
+            nk[s, k] = val(row[s], W + k)
+    U = np.zeros((n, W))
+    for j in range(n):
+        t = j % W
+        hits = [s for s in range(nslot) if row[s] == j]
+        assert len(hits) == 1, (j, hits)
+        sp = hits[0]
+        pv = np.array([w[sp, (t + k) % W] for k in range(W)])
+        U[j] = pv
+        for s in range(nslot):
+            m = w[s, t] / pv[0]
+            for k in range(1, W):
+                w[s, (t + k) % W] -= m * pv[k]
+            w[s, t] = nk[s, t]
+        nr = st["sched_next"][j]
+        row[sp] = nr
+        w[sp, :] = 0.0
+        nk[sp, :] = 0.0
+        for s in range(nslot):
+            nk[s, t] = val(row[s], j + PREFETCH_WINDOWS * W)
+    return U
+
+
+def test_kernel_schedule_emulation_reproduces_lu(small):
+    ops, g, prm = small
+    K, w = dense_ops(ops, prm, 1)
+    x = ops.initial(1)[:, 0]
+    mu = w @ (x / (ops.u @ x))
+    A = K - mu * ops.M
+    mask = (np.abs(sum(ops.FOp)) + np.abs(ops.FOp_g) + np.abs(ops.adv_raw) + np.abs(ops.M)) > 0
+    st = plan_structure(mask, g.num_p, g.num_sh, window_fn)
+    perm = permutation(g.num_p, g.num_sh)
+    Ap = A[np.ix_(perm, perm)]
+    U = emulate_factor(Ap, st)
+    # dense LU without pivoting
+    Ud = Ap.copy()
+    n = Ud.shape[0]
+    for j in range(n - 1):
+        l = Ud[j + 1:, j] / Ud[j, j]
+        Ud[j + 1:, j:] -= np.outer(l, Ud[j, j:])
+    for j in range(n):
+        ref = np.zeros(st["W"])
+        m = min(st["W"], n - j)
+        ref[:m] = Ud[j, j:j + m]
+        assert np.allclose(U[j], ref, rtol=1e-10, atol=1e-10 * np.abs(Ap).max())
+
+
+def test_plan_config2_fits_two_rows_per_lane():
+    ops, g, procs = oracle_batched(128, 2)
+    mask = (np.abs(sum(ops.FOp)) + np.abs(ops.FOp_g) + np.abs(ops.adv_raw) + np.abs(ops.M)) > 0
+    st = plan_structure(mask, g.num_p, g.num_sh, window_fn)
+    assert st["bw"] == 10 and st["W"] == 11
+    assert st["R"] == 2 and st["active_max"] <= 128
