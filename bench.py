@@ -30,7 +30,8 @@ XSEC = os.path.join(ROOT, "boltzmann_amd", "data", "eAr_synthetic_3proc.lxcat")
 B = 4096
 NR, LMAX, TE, EVMAX, Q = 128, 2, 4.0, 80.0, 5
 SEED = 20251202
-RTOL = 1e-12
+RTOL = 1e-10
+MAX_ITER = 12
 RATE_FLOOR = 1e-10
 
 
@@ -146,7 +147,7 @@ def main():
         x.copy_(x0)
         if ev is not None:
             ev[0].record(stream)
-        sc = solver.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=RTOL, max_iter=50, alpha_min=1e-16,
+        sc = solver.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=RTOL, max_iter=MAX_ITER, alpha_min=1e-16,
                               alpha_rho=0.5, ls_c=1e-4, mode=args.mode)
         if ev is not None:
             ev[1].record(stream)

@@ -21,6 +21,7 @@
 // Lane/slot mapping: each lane owns R row slots; a host-computed schedule assigns
 // rows to slots so that no two rows alive at the same step share a slot (a row is
 // alive from step first_col-2W, when its prefetch starts, to its pivot step).
+#include <algorithm>
 #include "bz_common.h"
 #include "../../include/bz.h"
 
@@ -53,7 +54,7 @@ struct BatchDev {
   double* x;                // (n, B) original order, in/out
   double* kprof;            // [B][P] scratch
   double* urows;            // [B][n][WP] scratch
-  double* vec;              // [B][4][n] scratch (A, Bv, P, WV)
+  double* vec;              // [B][4][n] scratch (row 0: growth vector of the instance)
   int* iters;               // [B]
   int* status;              // [B]  0 converged, 1 max_iter, 2 line search failed
   double* res_abs;          // [B]
@@ -89,26 +90,33 @@ __device__ void profile_matvec(const GridDev& g, const double* __restrict__ kp, 
   }
 }
 
-// In-place solve M z = b in the permuted order (per l-block banded Cholesky,
-// half bandwidth 3 in block index, stride ns); lanes 0..ns-1 each own one block.
-__device__ void mass_solve(const GridDev& g, double* __restrict__ b, int lane) {
-  const int ns = g.ns, npr = g.npr;
+// In-place solve M z = b in the permuted order: every l-block (stride ns) is the
+// reversed radial mass block with banded Cholesky rows lc[q*4 + d] = L[q, q-d]
+// (d = 0 entry stored inverted) held in LDS; lanes 0..ns-1 each own one block and
+// carry the last three solution values in registers.
+__device__ void mass_solve(int ns, int npr, const double* __restrict__ lc, double* __restrict__ b, int lane) {
   if (lane < ns) {
     const int l = lane;
+    double y1 = 0.0, y2 = 0.0, y3 = 0.0;          // y[q-1], y[q-2], y[q-3]
     for (int q = 0; q < npr; ++q) {
-      const int i = q * ns + l;
-      double s = b[i];
-      for (int d = 1; d <= 3 && q - d >= 0; ++d) s -= g.mchol[i * 4 + d] * b[i - d * ns];
-      b[i] = s / g.mchol[i * 4];
+      const double* L = lc + q * 4;
+      double s = b[q * ns + l];
+      s = fma(-L[1], y1, s);
+      s = fma(-L[2], y2, s);
+      s = fma(-L[3], y3, s);
+      const double y = s * L[0];
+      b[q * ns + l] = y;
+      y3 = y2; y2 = y1; y1 = y;
     }
+    double z1 = 0.0, z2 = 0.0, z3 = 0.0;          // z[q+1], z[q+2], z[q+3]
     for (int q = npr - 1; q >= 0; --q) {
-      const int i = q * ns + l;
-      double s = b[i];
-      for (int d = 1; d <= 3 && q + d < npr; ++d) {
-        const int k = i + d * ns;
-        s -= g.mchol[k * 4 + d] * b[k];
-      }
-      b[i] = s / g.mchol[i * 4];
+      double s = b[q * ns + l];
+      if (q + 1 < npr) s = fma(-lc[(q + 1) * 4 + 1], z1, s);
+      if (q + 2 < npr) s = fma(-lc[(q + 2) * 4 + 2], z2, s);
+      if (q + 3 < npr) s = fma(-lc[(q + 3) * 4 + 3], z3, s);
+      const double z = s * lc[q * 4];
+      b[q * ns + l] = z;
+      z3 = z2; z2 = z1; z1 = z;
     }
   }
 }
@@ -119,30 +127,69 @@ __device__ __forceinline__ double dot_lds(const double* __restrict__ a, const do
   return wave_sum(s);
 }
 
-// Norm of the projected residual r = (I - uhat uhat^T) (A + al*Bv - mu_a (X + al*P)).
-__device__ double proj_res_norm(const double* A, const double* Bv, const double* X, const double* P,
-                                const double* __restrict__ uhat, int n, double al, double mua, int lane) {
-  double su = 0.0, ss = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    const double R = (A[i] + al * Bv[i]) - mua * (X[i] + al * P[i]);
-    su = fma(uhat[i], R, su);
-    ss = fma(R, R, ss);
-  }
+// |(I - uhat uhat^T)(A - mu X)|  (two passes: explicit projection, no |R|^2 - (u.R)^2 cancellation)
+__device__ double proj_res_plain(const double* A, const double* X, const double* uh, int n, double mu, int lane) {
+  double su = 0.0;
+  for (int i = lane; i < n; i += 64) su = fma(uh[i], A[i] - mu * X[i], su);
   su = wave_sum(su);
-  // second pass for the explicit projection (avoids cancellation in |R|^2 - (u.R)^2)
   double s2 = 0.0;
   for (int i = lane; i < n; i += 64) {
-    const double R = (A[i] + al * Bv[i]) - mua * (X[i] + al * P[i]);
-    const double rp = R - su * uhat[i];
+    const double rp = (A[i] - mu * X[i]) - su * uh[i];
     s2 = fma(rp, rp, s2);
   }
-  (void)ss;
   return sqrt(wave_sum(s2));
+}
+
+// Residual norm at x + al*p for the Newton step y = c z1 + d z2, p = y - x, using
+// M^-1 K z1 = uhat + mu z1 and M^-1 K z2 = x + mu z2 (the two bordered solves):
+//   M^-1 K p = c uhat + d x + mu y - A,   A = M^-1 K x,
+//   R(al) = A + al M^-1 K p - mu(al) (x + al p).
+__device__ double proj_res_step(const double* A, const double* X, const double* Z1, const double* Z2,
+                                const double* uh, int n, double c, double d, double mu, double al, double mua,
+                                int lane) {
+  double su = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double y = fma(c, Z1[i], d * Z2[i]);
+    const double p = y - X[i];
+    const double kp = fma(c, uh[i], fma(d, X[i], fma(mu, y, -A[i])));
+    const double R = fma(al, kp, A[i]) - mua * fma(al, p, X[i]);
+    su = fma(uh[i], R, su);
+  }
+  su = wave_sum(su);
+  double s2 = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double y = fma(c, Z1[i], d * Z2[i]);
+    const double p = y - X[i];
+    const double kp = fma(c, uh[i], fma(d, X[i], fma(mu, y, -A[i])));
+    const double R = fma(al, kp, A[i]) - mua * fma(al, p, X[i]);
+    const double rp = R - su * uh[i];
+    s2 = fma(rp, rp, s2);
+  }
+  return sqrt(wave_sum(s2));
+}
+
+// y = M x from the M profile (M is l-block diagonal, radial 7-band: columns i + d*ns, |d| <= 3).
+__device__ void mass_matvec(const GridDev& g, const double* __restrict__ mprof, const int* __restrict__ fcol,
+                            const int* __restrict__ poff, const double* __restrict__ x, double* __restrict__ y,
+                            int lane) {
+  const int n = g.n, ns = g.ns;
+  for (int i = lane; i < n; i += 64) {
+    const int base = poff[i] - fcol[i];
+    double s = 0.0;
+#pragma unroll
+    for (int d = -3; d <= 3; ++d) {
+      const int c = i + d * ns;
+      if (c >= 0 && c < n) s = fma(mprof[base + c], x[c], s);
+    }
+    y[i] = s;
+  }
 }
 
 // Factorise A = K - mu M (profile, permuted order, no pivoting) and forward-eliminate
 // the two right-hand sides b1 = M uhat, b2 = M x.  Writes U rows (1/U_jj first) to
 // urows and the eliminated right-hand sides to Y1, Y2 (LDS, indexed by pivot row).
+// The column prefetch (2W steps ahead) is branch-free so that the memory-counter
+// waits can be counted statically: every step issues exactly 2R loads and one store.
 template <int W, int R>
 __device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof, double mu,
                                const int* __restrict__ s_fcol, const int* __restrict__ s_poff,
@@ -152,6 +199,7 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   const int n = g.n, bw = g.bw;
   const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
   int row[R], fc[R], lc[R], off[R];
+  unsigned vmask[R];
   double w[R][W], nk[R][W], nm[R][W], b1[R], b2[R];
 
 #pragma unroll
@@ -167,12 +215,13 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
     } else {
       fc[r] = 0x7fffffff; lc[r] = -1; off[r] = 0; b1[r] = 0.0; b2[r] = 0.0;
     }
+    vmask[r] = 0u;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       const int c0 = k, c1 = W + k;
       double v0 = 0.0, a1 = 0.0, m1 = 0.0;
       if (c0 >= fc[r] && c0 <= lc[r]) v0 = kprof[off[r] + c0] - mu * mprof[off[r] + c0];
-      if (c1 >= fc[r] && c1 <= lc[r]) { a1 = kprof[off[r] + c1]; m1 = mprof[off[r] + c1]; }
+      if (c1 >= fc[r] && c1 <= lc[r]) { a1 = kprof[off[r] + c1]; m1 = mprof[off[r] + c1]; vmask[r] |= 1u << k; }
       w[r][k] = v0; nk[r][k] = a1; nm[r][k] = m1;
     }
   }
@@ -193,8 +242,9 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           if (hit == r) {
+            s_piv[0] = fast_rcp(w[r][t]);
 #pragma unroll
-            for (int k = 0; k < W; ++k) s_piv[k] = w[r][(t + k) % W];
+            for (int k = 1; k < W; ++k) s_piv[k] = w[r][(t + k) % W];
             s_piv[W] = b1[r];
             s_piv[W + 1] = b2[r];
           }
@@ -207,16 +257,16 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
 #pragma unroll
       for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
       const double pb1 = s_piv[W], pb2 = s_piv[W + 1];
-      const double rinv = fast_rcp(pv[0]);
-      // --- store U row j and the eliminated rhs
+      const double rinv = pv[0];                     // published as 1/U_jj
+      // --- U row j (1/U_jj, U_j,j+1..) straight from the published row, and the rhs;
+      // every lane stores entry lane % WP (same values, no branch; entry W is padding)
       {
-        double v = rinv;
-#pragma unroll
-        for (int k = 1; k < W; ++k) v = (lane == k) ? pv[k] : v;
-        if (lane < W) urows[(size_t)j * WP + lane] = v;
-        if (lane == 0) { Y1[j] = pb1; Y2[j] = pb2; }
+        const int kk = lane % WP;
+        urows[(size_t)j * WP + kk] = s_piv[kk];
+        Y1[j] = pb1;
+        Y2[j] = pb2;
       }
-      // --- eliminate column j, shift in column j+W, prefetch column j+2W
+      // --- eliminate column j, shift in column j+W (prefetched 2W steps ago)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double m = w[r][t] * rinv;
@@ -224,17 +274,19 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
         for (int k = 1; k < W; ++k) w[r][(t + k) % W] = fma(-m, pv[k], w[r][(t + k) % W]);
         b1[r] = fma(-m, pb1, b1[r]);
         b2[r] = fma(-m, pb2, b2[r]);
-        w[r][t] = fma(-mu, nm[r][t], nk[r][t]);
+        const double v = fma(-mu, nm[r][t], nk[r][t]);
+        w[r][t] = ((vmask[r] >> t) & 1u) ? v : 0.0;
       }
-      // --- retire the pivot row; its slot takes the next scheduled row
+      // --- retire the pivot row; its slot takes the next scheduled row (LDS reads only)
       if (lane == lp) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           if (hit == r) {
             const int nr = s_next[row[r]];
             row[r] = nr;
+            vmask[r] = 0u;
 #pragma unroll
-            for (int k = 0; k < W; ++k) { w[r][k] = 0.0; nk[r][k] = 0.0; nm[r][k] = 0.0; }
+            for (int k = 0; k < W; ++k) w[r][k] = 0.0;
             if (nr >= 0) {
               fc[r] = s_fcol[nr];
               lc[r] = min(nr + bw, n - 1);
@@ -247,44 +299,125 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
           }
         }
       }
+      // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int c = j + 2 * W;
-        double a = 0.0, m2 = 0.0;
-        if (c >= fc[r] && c <= lc[r]) { a = kprof[off[r] + c]; m2 = mprof[off[r] + c]; }
-        nk[r][t] = a;
-        nm[r][t] = m2;
+        const bool valid = (c >= fc[r]) && (c <= lc[r]);
+        const int idx = valid ? off[r] + c : 0;
+        nk[r][t] = kprof[idx];
+        nm[r][t] = mprof[idx];
+        vmask[r] = (vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
       }
     }
   }
 }
 
 // Back substitution U z = y for the two right-hand sides held in Y1/Y2 (in place).
-// Lane 0 solves Y1, lane 1 solves Y2; U rows are read as broadcasts.
+// U rows are staged through LDS in chunks of 2W rows: all 64 lanes load the next chunk
+// (contiguous, coalesced) into registers while lanes 0 and 1 (one right-hand side each)
+// sweep the current chunk from LDS; the window z[] holds z_{j+1..j+W-1} at static
+// positions j mod W (chunk bases are multiples of W).
 template <int W>
 __device__ void back_substitute(int n, const double* __restrict__ urows, int WP, double* __restrict__ Y1,
-                                double* __restrict__ Y2, int lane) {
-  if (lane < 2) {
-    double* Y = lane == 0 ? Y1 : Y2;
-    double z[W];
+                                double* __restrict__ Y2, double* __restrict__ stg, int lane) {
+  constexpr int CH = 2 * W;
+  constexpr int PER = (CH * ((W + 1) & ~1) + 63) / 64;   // doubles per lane per chunk
+  const int csize = CH * WP;
+  double* Y = lane == 0 ? Y1 : Y2;
+  double z[W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) z[k] = 0.0;
-    const int jtop = ((n + W - 1) / W) * W;
-    for (int j0 = jtop - W; j0 >= 0; j0 -= W) {
+  for (int k = 0; k < W; ++k) z[k] = 0.0;
+  const int nch = (n + CH - 1) / CH;
+  double buf[PER];
+  // prime: chunk nch-1
+  {
+    const int base = (nch - 1) * CH * WP;
+    const int lim = n * WP;
 #pragma unroll
-      for (int tt = W - 1; tt >= 0; --tt) {
+    for (int e = 0; e < PER; ++e) {
+      const int o = lane + 64 * e;
+      const int gi = base + o;
+      buf[e] = urows[(o < csize && gi < lim) ? gi : 0];
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int o = lane + 64 * e;
+      if (o < csize) stg[o] = buf[e];
+    }
+  }
+  __syncthreads();
+  for (int c = nch - 1; c >= 0; --c) {
+    // issue the loads of chunk c-1
+    if (c > 0) {
+      const int base = (c - 1) * CH * WP;
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int o = lane + 64 * e;
+        buf[e] = urows[base + (o < csize ? o : 0)];
+      }
+    }
+    if (lane < 2) {
+      const int j0 = c * CH;
+#pragma unroll
+      for (int tt = CH - 1; tt >= 0; --tt) {
         const int j = j0 + tt;
         if (j < n) {
-          const double* ur = urows + (size_t)j * WP;
+          const double* ur = stg + tt * WP;
           double s = Y[j];
 #pragma unroll
           for (int k = W - 1; k >= 1; --k) s = fma(-ur[k], z[(tt + k) % W], s);
           const double zj = s * ur[0];
-          z[tt] = zj;
+          z[tt % W] = zj;
           Y[j] = zj;
         }
       }
     }
+    __syncthreads();
+    if (c > 0) {
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int o = lane + 64 * e;
+        if (o < csize) stg[o] = buf[e];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Per-instance K profile (sum of the scaled operators) and growth vector, all instances at once.
+__global__ void k_prepare0d(GridDev g, BatchDev b) {
+  const long total = (long)b.B * g.P;
+  const int nk = g.nop - 1;
+  const int S = g.S;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int inst = (int)(idx / g.P);
+    const int e = (int)(idx - (long)inst * g.P);
+    const double n0 = b.n0[inst];
+    double v = 0.0;
+    for (int m = 0; m < nk; ++m) {
+      double cm;
+      if (m < S) cm = n0 * b.ns_by_n0[(size_t)m * b.B + inst];
+      else if (m == S) cm = n0 * b.Tg[inst] * b.tg_scale;
+      else cm = b.E[inst];
+      v = fma(cm, g.opprof[(size_t)m * g.P + e], v);
+    }
+    b.kprof[idx] = v;
+  }
+  const long totw = (long)b.B * g.n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < totw; idx += (long)gridDim.x * blockDim.x) {
+    const int inst = (int)(idx / g.n);
+    const int i = (int)(idx - (long)inst * g.n);
+    const double n0 = b.n0[inst];
+    double v = 0.0;
+    for (int m = 0; m < nk; ++m) {
+      double cm;
+      if (m < S) cm = n0 * b.ns_by_n0[(size_t)m * b.B + inst];
+      else if (m == S) cm = n0 * b.Tg[inst] * b.tg_scale;
+      else cm = b.E[inst];
+      v = fma(cm, g.wsp[(size_t)m * g.n + i], v);
+    }
+    b.vec[(size_t)inst * 4 * g.n + i] = v;
   }
 }
 
@@ -298,127 +431,133 @@ __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
   double* Z2 = Z1 + n;
   double* MXv = Z2 + n;
   double* MUH = MXv + n;
-  double* PIV = MUH + n;                                   // W + 2 (padded to 32)
-  int* FC = reinterpret_cast<int*>(PIV + 32);
+  double* AV = MUH + n;                                    // M^-1 K x
+  double* WV = AV + n;                                     // growth vector w (mu = w.x)
+  double* UH = WV + n;                                     // uhat
+  double* PIV = UH + n;                                    // W + 2 (padded to 32)
+  double* LCH = PIV + 32;                                  // npr x 4 mass Cholesky rows
+  double* STG = LCH + 4 * g.npr;                           // 2W x WP back-substitution stage
+  int* FC = reinterpret_cast<int*>(STG + 2 * W * WP);
   int* PO = FC + n;
   int* NX = PO + (n + 1);
 
-  double* A = b.vec + (size_t)inst * 4 * n;                // M^-1 K x
-  double* Bv = A + n;                                      // M^-1 K p
-  double* Pv = Bv + n;                                     // Newton step
-  double* WV = Pv + n;                                     // growth vector w (mu = w.x)
-  double* kprof = b.kprof + (size_t)inst * g.P;
+  const double* __restrict__ kprof = b.kprof + (size_t)inst * g.P;
+  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
   double* urows = b.urows + (size_t)inst * n * WP;
+  const double* wv_g = b.vec + (size_t)inst * 4 * n;
 
-  // ---------------- per-grid tables into LDS
+  // ---------------- tables into LDS
   for (int i = lane; i < n; i += 64) {
     FC[i] = g.first_col[i];
     NX[i] = g.sched_next[i];
     MUH[i] = g.muhat[i];
+    UH[i] = g.uhat[i];
+    WV[i] = wv_g[i];
+    X[i] = b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst];
   }
   for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
-
-  // ---------------- per-instance coefficients and K profile
-  const int S = g.S;
-  double coef[kMaxOps];
-  const double n0 = b.n0[inst];
-#pragma unroll
-  for (int m = 0; m < kMaxOps; ++m) coef[m] = 0.0;
-  for (int s = 0; s < S; ++s) coef[s] = n0 * b.ns_by_n0[(size_t)s * b.B + inst];
-  coef[S] = n0 * b.Tg[inst] * b.tg_scale;
-  coef[S + 1] = b.E[inst];
-  const int nk = g.nop - 1;
-  for (int e = lane; e < g.P; e += 64) {
-    double v = 0.0;
-    for (int m = 0; m < nk; ++m) v = fma(coef[m], g.opprof[(size_t)m * g.P + e], v);
-    kprof[e] = v;
+  for (int e = lane; e < 4 * g.npr; e += 64) {
+    const int q = e >> 2, d = e & 3;
+    const double v = g.mchol[(size_t)(q * g.ns) * 4 + d];
+    LCH[e] = (d == 0) ? 1.0 / v : ((q - d >= 0) ? v : 0.0);
   }
-  for (int i = lane; i < n; i += 64) {
-    double v = 0.0;
-    for (int m = 0; m < nk; ++m) v = fma(coef[m], g.wsp[(size_t)m * n + i], v);
-    WV[i] = v;
-  }
-  // initial iterate (original order, (n, B) layout) -> permuted, normalised u.x = 1
-  for (int i = lane; i < n; i += 64) X[i] = b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst];
   __syncthreads();
   {
-    const double ux = dot_lds(g.u, X, n, lane);
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s = fma(g.u[i], X[i], s);
+    const double ux = wave_sum(s);
     for (int i = lane; i < n; i += 64) X[i] = X[i] / ux;
   }
   __syncthreads();
 
-  // ---------------- initial residual
-  profile_matvec(g, kprof, FC, PO, X, A, lane);
+  // ---------------- initial residual A = M^-1 K x
+  profile_matvec(g, kprof, FC, PO, X, AV, lane);
   __syncthreads();
-  mass_solve(g, A, lane);
+  mass_solve(g.ns, g.npr, LCH, AV, lane);
   __syncthreads();
   double mu = dot_lds(WV, X, n, lane);
-  for (int i = lane; i < n; i += 64) { Pv[i] = 0.0; Bv[i] = 0.0; }
-  __syncthreads();
-  const double r0 = proj_res_norm(A, Bv, X, Pv, g.uhat, n, 0.0, mu, lane);
+  const double r0 = proj_res_plain(AV, X, UH, n, mu, lane);
   double rcur = r0;
   double abs_err = r0;
   const double tol = b.atol + b.rtol * r0;
-  int it = 0, status = 1;
+  int it = 0, status = 1, verify = 0;
 
-  while (it < b.max_iter) {
-    if (b.mode == 0) { if (!(abs_err > tol)) { status = 0; break; } abs_err = rcur; }
-    else { if (!(rcur > tol)) { status = 0; break; } }
-    mu = dot_lds(WV, X, n, lane);
-    // M x -> MXv : rows of M are the last operator's profile
-    {
-      const double* mp = g.opprof + (size_t)(g.nop - 1) * g.P;
-      profile_matvec(g, mp, FC, PO, X, MXv, lane);
+  for (;;) {
+    bool done = false;
+    if (b.mode == 0) { if (!(abs_err > tol)) { status = 0; done = true; } }
+    else if (!(rcur > tol)) { status = 0; done = true; }
+    if (!done && it >= b.max_iter) { status = 1; done = true; }
+    if (done) {
+      // recompute the residual from scratch; iterate on if the incremental value drifted
+      profile_matvec(g, kprof, FC, PO, X, MXv, lane);
+      __syncthreads();
+      mass_solve(g.ns, g.npr, LCH, MXv, lane);
+      __syncthreads();
+      for (int i = lane; i < n; i += 64) AV[i] = MXv[i];
+      __syncthreads();
+      mu = dot_lds(WV, X, n, lane);
+      const double rtrue = proj_res_plain(AV, X, UH, n, mu, lane);
+      rcur = rtrue;
+      if (status == 0 && rtrue > tol && it < b.max_iter && verify < 2) {
+        ++verify;
+        abs_err = rtrue;
+        status = 1;
+      } else {
+        if (status == 0 && rtrue > tol) status = 1;
+        break;
+      }
     }
+    if (b.mode == 0) abs_err = rcur;
+    mu = dot_lds(WV, X, n, lane);
+    mass_matvec(g, mprof, FC, PO, X, MXv, lane);
     __syncthreads();
     factor_forward<W, R>(g, kprof, mu, FC, PO, NX, MUH, MXv, Z1, Z2, PIV, urows, WP, lane);
     __syncthreads();
-    back_substitute<W>(n, urows, WP, Z1, Z2, lane);
+    back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
     // --- 2x2 system for (c, d)
-    const double s11 = dot_lds(g.uhat, Z1, n, lane);
-    const double s12 = dot_lds(g.uhat, Z2, n, lane);
-    const double ux = dot_lds(g.uhat, X, n, lane);
+    const double s11 = dot_lds(UH, Z1, n, lane);
+    const double s12 = dot_lds(UH, Z2, n, lane);
+    const double ux = dot_lds(UH, X, n, lane);
+    const double wz1 = dot_lds(WV, Z1, n, lane);
+    const double wz2 = dot_lds(WV, Z2, n, lane);
     double c, d;
     if (b.mode == 0) {
       c = ux / s11;
       d = 0.0;
     } else {
-      const double s21 = dot_lds(WV, Z1, n, lane);
-      const double s22 = dot_lds(WV, Z2, n, lane) - 1.0;
-      const double det = s11 * s22 - s12 * s21;
+      const double s22 = wz2 - 1.0;
+      const double det = s11 * s22 - s12 * wz1;
       c = (ux * s22 - s12 * mu) / det;
-      d = (s11 * mu - s21 * ux) / det;
+      d = (s11 * mu - wz1 * ux) / det;
     }
-    for (int i = lane; i < n; i += 64) Pv[i] = fma(c, Z1[i], fma(d, Z2[i], -X[i]));
-    __syncthreads();
-    // --- b = M^-1 K p
-    profile_matvec(g, kprof, FC, PO, Pv, Bv, lane);
-    __syncthreads();
-    mass_solve(g, Bv, lane);
-    __syncthreads();
+    const double mup = c * wz1 + d * wz2 - mu;         // w.p
     // --- Armijo backtracking on 0.5 |r|^2 (bte_0d3v_batched.py:936-952)
-    const double mup = dot_lds(WV, Pv, n, lane);
     const double ae2 = rcur * rcur;
     double alpha = 1.0;
     bool diverged = false;
-    double rnew = proj_res_norm(A, Bv, X, Pv, g.uhat, n, alpha, mu + alpha * mup, lane);
+    double rnew = proj_res_step(AV, X, Z1, Z2, UH, n, c, d, mu, alpha, mu + alpha * mup, lane);
     while (0.5 * rnew * rnew > 0.5 * ae2 - b.ls_c * alpha * ae2) {
       alpha *= b.alpha_rho;
       if (alpha < b.alpha_min) { diverged = true; break; }
-      rnew = proj_res_norm(A, Bv, X, Pv, g.uhat, n, alpha, mu + alpha * mup, lane);
+      rnew = proj_res_step(AV, X, Z1, Z2, UH, n, c, d, mu, alpha, mu + alpha * mup, lane);
     }
-    if (diverged) { status = 2; break; }
+    if (diverged) {
+      status = 2;
+      break;
+    }
     for (int i = lane; i < n; i += 64) {
-      X[i] = fma(alpha, Pv[i], X[i]);
-      A[i] = fma(alpha, Bv[i], A[i]);
+      const double y = fma(c, Z1[i], d * Z2[i]);
+      const double p = y - X[i];
+      const double kp = fma(c, UH[i], fma(d, X[i], fma(mu, y, -AV[i])));
+      X[i] = fma(alpha, p, X[i]);
+      AV[i] = fma(alpha, kp, AV[i]);
     }
     __syncthreads();
     rcur = rnew;
     if (b.mode == 0 && (it % 10) == 0) abs_err = rcur;
     ++it;
   }
-  if (status == 1 && ((b.mode == 0 && !(abs_err > tol)) || (b.mode != 0 && !(rcur > tol)))) status = 0;
 
   // ---------------- write back (original order)
   for (int i = lane; i < n; i += 64) b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst] = X[i];
@@ -432,8 +571,14 @@ __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
 
 template <int W, int R>
 int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream) {
-  const size_t lds = (size_t)(5 * g.n + 32) * sizeof(double) + (size_t)(3 * g.n + 1) * sizeof(int);
+  const int WP = (W + 1) & ~1;
+  const size_t lds = (size_t)(8 * g.n + 32 + 4 * g.npr + 2 * W * WP) * sizeof(double) +
+                     (size_t)(3 * g.n + 1) * sizeof(int);
   if (lds > 64 * 1024) return BZ_EINVAL;
+  const long work = (long)b.B * g.P;
+  const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_prepare0d, dim3(blocks), dim3(256), 0, stream, g, b);
+  BZ_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_newton0d<W, R>), dim3(b.B), dim3(64), lds, stream, g, b);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
