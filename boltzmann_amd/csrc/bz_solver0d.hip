@@ -16,11 +16,13 @@
 // Data layout (all vectors in the permuted order i' = (np_r-1-p)*ns + l):
 //   * shared per grid: operator values in row profile storage (columns
 //     first_col[i]..min(i+bw, n-1)), nop arrays: C_0..C_{S-1}, C_Tg, A_E, M
-//   * per instance: K profile (sum of the scaled operators, built once per solve),
-//     U rows (W doubles each, 1/U_jj first), scalar results.
-// Lane/slot mapping: each lane owns R row slots; a host-computed schedule assigns
-// rows to slots so that no two rows alive at the same step share a slot (a row is
-// alive from step first_col-2W, when its prefetch starts, to its pivot step).
+//   * per instance: K profile (sum of the scaled operators, built for the whole
+//     batch by k_prepare0d), U rows (WP doubles each, 1/U_jj first).
+// Lane/slot mapping: slot s lives in lane s % 64, register level s / 64.  A host
+// schedule assigns rows to slots so that no two rows alive at the same step share
+// a slot (a row is alive from step first_col - 2W, when its prefetch starts, to its
+// pivot step) and uses level 1+ only where more than 64 rows are alive; every
+// W-step chunk runs only the levels its rows occupy.
 #include <algorithm>
 #include "bz_common.h"
 #include "../../include/bz.h"
@@ -43,6 +45,8 @@ struct GridDev {
   const double* u;          // [n]
   const double* wsp;        // [nop-1][n]  W_m = Op_m^T M^-1 u (zero rows: not in the growth term)
   const double* mchol;      // [n][4]  banded Cholesky of M (permuted), L[i, i-d*ns]
+  const int* chunk_levels;  // [ceil(n/W)] register levels alive in each W-step chunk
+  const unsigned* ent_rc;   // [P] (row << 16) | column of every profile entry
 };
 
 struct BatchDev {
@@ -76,28 +80,26 @@ __device__ __forceinline__ double fast_rcp(double a) {
   return r;
 }
 
-// y = K x  (profile storage of this instance), result into y (LDS or global), lane-strided rows
-__device__ void profile_matvec(const GridDev& g, const double* __restrict__ kp, const int* __restrict__ fcol,
-                               const int* __restrict__ poff, const double* __restrict__ x, double* __restrict__ y,
-                               int lane) {
-  for (int i = lane; i < g.n; i += 64) {
-    const int f = fcol[i];
-    const int o = poff[i];
-    const int len = poff[i + 1] - o;
-    double s = 0.0;
-    for (int e = 0; e < len; ++e) s = fma(kp[o + e], x[f + e], s);
-    y[i] = s;
+// y = K x over the flat profile: entries are lane-strided (coalesced loads, no
+// dependence between iterations) and accumulated into y[row] with LDS atomics.
+// y must be zeroed and synchronised by the caller.
+__device__ void flat_matvec(int P, const double* __restrict__ kp, const unsigned* __restrict__ rc,
+                            const double* __restrict__ x, double* y, int lane) {
+  for (int e = lane; e < P; e += 64) {
+    const unsigned v = rc[e];
+    const int row = (int)(v >> 16), col = (int)(v & 0xffffu);
+    atomicAdd(&y[row], kp[e] * x[col]);
   }
 }
 
 // In-place solve M z = b in the permuted order: every l-block (stride ns) is the
 // reversed radial mass block with banded Cholesky rows lc[q*4 + d] = L[q, q-d]
-// (d = 0 entry stored inverted) held in LDS; lanes 0..ns-1 each own one block and
-// carry the last three solution values in registers.
+// (d = 0 entry stored inverted, out-of-band entries zero) held in LDS; lanes
+// 0..ns-1 each own one block and carry the last three values in registers.
 __device__ void mass_solve(int ns, int npr, const double* __restrict__ lc, double* __restrict__ b, int lane) {
   if (lane < ns) {
     const int l = lane;
-    double y1 = 0.0, y2 = 0.0, y3 = 0.0;          // y[q-1], y[q-2], y[q-3]
+    double y1 = 0.0, y2 = 0.0, y3 = 0.0;
     for (int q = 0; q < npr; ++q) {
       const double* L = lc + q * 4;
       double s = b[q * ns + l];
@@ -108,7 +110,7 @@ __device__ void mass_solve(int ns, int npr, const double* __restrict__ lc, doubl
       b[q * ns + l] = y;
       y3 = y2; y2 = y1; y1 = y;
     }
-    double z1 = 0.0, z2 = 0.0, z3 = 0.0;          // z[q+1], z[q+2], z[q+3]
+    double z1 = 0.0, z2 = 0.0, z3 = 0.0;
     for (int q = npr - 1; q >= 0; --q) {
       double s = b[q * ns + l];
       if (q + 1 < npr) s = fma(-lc[(q + 1) * 4 + 1], z1, s);
@@ -121,14 +123,15 @@ __device__ void mass_solve(int ns, int npr, const double* __restrict__ lc, doubl
   }
 }
 
-__device__ __forceinline__ double dot_lds(const double* __restrict__ a, const double* __restrict__ b, int n, int lane) {
+__device__ __forceinline__ double dot(const double* __restrict__ a, const double* __restrict__ b, int n, int lane) {
   double s = 0.0;
   for (int i = lane; i < n; i += 64) s = fma(a[i], b[i], s);
   return wave_sum(s);
 }
 
 // |(I - uhat uhat^T)(A - mu X)|  (two passes: explicit projection, no |R|^2 - (u.R)^2 cancellation)
-__device__ double proj_res_plain(const double* A, const double* X, const double* uh, int n, double mu, int lane) {
+__device__ double proj_res_plain(const double* A, const double* X, const double* __restrict__ uh, int n, double mu,
+                                 int lane) {
   double su = 0.0;
   for (int i = lane; i < n; i += 64) su = fma(uh[i], A[i] - mu * X[i], su);
   su = wave_sum(su);
@@ -145,8 +148,8 @@ __device__ double proj_res_plain(const double* A, const double* X, const double*
 //   M^-1 K p = c uhat + d x + mu y - A,   A = M^-1 K x,
 //   R(al) = A + al M^-1 K p - mu(al) (x + al p).
 __device__ double proj_res_step(const double* A, const double* X, const double* Z1, const double* Z2,
-                                const double* uh, int n, double c, double d, double mu, double al, double mua,
-                                int lane) {
+                                const double* __restrict__ uh, int n, double c, double d, double mu, double al,
+                                double mua, int lane) {
   double su = 0.0;
   for (int i = lane; i < n; i += 64) {
     const double y = fma(c, Z1[i], d * Z2[i]);
@@ -169,12 +172,11 @@ __device__ double proj_res_step(const double* A, const double* X, const double* 
 }
 
 // y = M x from the M profile (M is l-block diagonal, radial 7-band: columns i + d*ns, |d| <= 3).
-__device__ void mass_matvec(const GridDev& g, const double* __restrict__ mprof, const int* __restrict__ fcol,
+__device__ void mass_matvec(int n, int ns, const double* __restrict__ mprof, const int* __restrict__ first_col,
                             const int* __restrict__ poff, const double* __restrict__ x, double* __restrict__ y,
                             int lane) {
-  const int n = g.n, ns = g.ns;
   for (int i = lane; i < n; i += 64) {
-    const int base = poff[i] - fcol[i];
+    const int base = poff[i] - first_col[i];
     double s = 0.0;
 #pragma unroll
     for (int d = -3; d <= 3; ++d) {
@@ -185,130 +187,154 @@ __device__ void mass_matvec(const GridDev& g, const double* __restrict__ mprof, 
   }
 }
 
-// Factorise A = K - mu M (profile, permuted order, no pivoting) and forward-eliminate
-// the two right-hand sides b1 = M uhat, b2 = M x.  Writes U rows (1/U_jj first) to
-// urows and the eliminated right-hand sides to Y1, Y2 (LDS, indexed by pivot row).
-// The column prefetch (2W steps ahead) is branch-free so that the memory-counter
-// waits can be counted statically: every step issues exactly 2R loads and one store.
+// Register state of the factorisation: R levels of row slots per lane.
 template <int W, int R>
-__device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof, double mu,
-                               const int* __restrict__ s_fcol, const int* __restrict__ s_poff,
-                               const int* __restrict__ s_next, const double* __restrict__ s_muhat,
-                               const double* __restrict__ s_mx, double* __restrict__ Y1, double* __restrict__ Y2,
-                               double* __restrict__ s_piv, double* __restrict__ urows, int WP, int lane) {
-  const int n = g.n, bw = g.bw;
-  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+struct FactorState {
   int row[R], fc[R], lc[R], off[R];
   unsigned vmask[R];
-  double w[R][W], nk[R][W], nm[R][W], b1[R], b2[R];
+  double w[R][W], nk[R][W], b1[R], b2[R];
+};
 
+// One W-step chunk of the factorisation running levels 0..RA-1 only.
+template <int W, int R, int RA>
+__device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
+                                             const double* __restrict__ kprof,
+                                             const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
+                                             const short* __restrict__ s_next, const double* __restrict__ muhat,
+                                             double* __restrict__ Y1, double* __restrict__ Y2,
+                                             double* __restrict__ s_piv, double* __restrict__ urows, int WP,
+                                             int lane) {
+#pragma unroll
+  for (int t = 0; t < W; ++t) {
+    const int j = j0 + t;
+    if (j >= n) break;
+    // --- locate the pivot row j
+    int hit = -1;
+#pragma unroll
+    for (int r = 0; r < RA; ++r) hit = (s.row[r] == j) ? r : hit;
+    const unsigned long long bal = __ballot(hit >= 0);
+    const int lp = __builtin_amdgcn_readfirstlane(__ffsll((long long)bal) - 1);
+    // --- pivot lane publishes 1/U_jj, U_j,j+1..j+W-1 and the two rhs values
+    if (lane == lp) {
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        if (hit == r) {
+          s_piv[0] = fast_rcp(s.w[r][t]);
+#pragma unroll
+          for (int k = 1; k < W; ++k) s_piv[k] = s.w[r][(t + k) % W];
+          s_piv[W] = s.b1[r];
+          s_piv[W + 1] = s.b2[r];
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double pv[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
+    const double pb1 = s_piv[W], pb2 = s_piv[W + 1];
+    const double rinv = pv[0];
+    // --- U row j and the eliminated rhs; every lane stores entry lane % WP (no branch)
+    {
+      const int kk = lane % WP;
+      urows[(size_t)j * WP + kk] = s_piv[kk];
+      Y1[j] = pb1;
+      Y2[j] = pb2;
+    }
+    // --- eliminate column j, shift in column j+W (prefetched 2W steps ago)
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const double m = s.w[r][t] * rinv;
+#pragma unroll
+      for (int k = 1; k < W; ++k) s.w[r][(t + k) % W] = fma(-m, pv[k], s.w[r][(t + k) % W]);
+      s.b1[r] = fma(-m, pb1, s.b1[r]);
+      s.b2[r] = fma(-m, pb2, s.b2[r]);
+      s.w[r][t] = ((s.vmask[r] >> t) & 1u) ? s.nk[r][t] : 0.0;
+    }
+    // --- retire the pivot row; its slot takes the next scheduled row (LDS reads only)
+    if (lane == lp) {
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        if (hit == r) {
+          const int nr = s_next[s.row[r]];
+          s.row[r] = nr;
+          s.vmask[r] = 0u;
+#pragma unroll
+          for (int k = 0; k < W; ++k) s.w[r][k] = 0.0;
+          if (nr >= 0) {
+            s.fc[r] = s_fcol[nr];
+            s.lc[r] = min(nr + bw, n - 1);
+            s.off[r] = s_poff[nr] - s.fc[r];
+            s.b1[r] = muhat[nr];
+            s.b2[r] = Y2[nr];
+          } else {
+            s.fc[r] = 0x7fffffff; s.lc[r] = -1; s.off[r] = 0; s.b1[r] = 0.0; s.b2[r] = 0.0;
+          }
+        }
+      }
+    }
+    // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked)
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int c = j + 2 * W;
+      const bool valid = (c >= s.fc[r]) && (c <= s.lc[r]);
+      const int idx = valid ? s.off[r] + c : 0;
+      s.nk[r][t] = kprof[idx];
+      s.vmask[r] = (s.vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
+    }
+  }
+}
+
+// Factorise A = K - mu M (the instance profile already holds K - mu M; permuted order,
+// no pivoting) and forward-eliminate
+// the two right-hand sides b1 = M uhat (rows read from Y1 = M uhat on entry) and
+// b2 = M x (read from Y2 on entry).  Y1[j], Y2[j] are overwritten by the eliminated
+// values at pivot step j, which is never before the entry of row j.  U rows (1/U_jj
+// first) go to urows.  The column prefetch is branch-free so that every step issues
+// a fixed number of loads and the memory-counter waits can be counted statically.
+template <int W, int R>
+__device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof,
+                               const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
+                               const short* __restrict__ s_next, const short* __restrict__ s_lev,
+                               double* __restrict__ Y1, double* __restrict__ Y2, double* __restrict__ s_piv,
+                               double* __restrict__ urows, int WP, int lane) {
+  const int n = g.n, bw = g.bw;
+  FactorState<W, R> s;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int rw = g.slot_first[lane * R + r];
-    row[r] = rw;
+    const int rw = g.slot_first[r * 64 + lane];
+    s.row[r] = rw;
     if (rw >= 0) {
-      fc[r] = s_fcol[rw];
-      lc[r] = min(rw + bw, n - 1);
-      off[r] = s_poff[rw] - fc[r];
-      b1[r] = s_muhat[rw];
-      b2[r] = s_mx[rw];
+      s.fc[r] = s_fcol[rw];
+      s.lc[r] = min(rw + bw, n - 1);
+      s.off[r] = s_poff[rw] - s.fc[r];
+      s.b1[r] = Y1[rw];
+      s.b2[r] = Y2[rw];
     } else {
-      fc[r] = 0x7fffffff; lc[r] = -1; off[r] = 0; b1[r] = 0.0; b2[r] = 0.0;
+      s.fc[r] = 0x7fffffff; s.lc[r] = -1; s.off[r] = 0; s.b1[r] = 0.0; s.b2[r] = 0.0;
     }
-    vmask[r] = 0u;
+    s.vmask[r] = 0u;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       const int c0 = k, c1 = W + k;
-      double v0 = 0.0, a1 = 0.0, m1 = 0.0;
-      if (c0 >= fc[r] && c0 <= lc[r]) v0 = kprof[off[r] + c0] - mu * mprof[off[r] + c0];
-      if (c1 >= fc[r] && c1 <= lc[r]) { a1 = kprof[off[r] + c1]; m1 = mprof[off[r] + c1]; vmask[r] |= 1u << k; }
-      w[r][k] = v0; nk[r][k] = a1; nm[r][k] = m1;
+      double v0 = 0.0, a1 = 0.0;
+      if (c0 >= s.fc[r] && c0 <= s.lc[r]) v0 = kprof[s.off[r] + c0];
+      if (c1 >= s.fc[r] && c1 <= s.lc[r]) {
+        a1 = kprof[s.off[r] + c1];
+        s.vmask[r] |= 1u << k;
+      }
+      s.w[r][k] = v0; s.nk[r][k] = a1;
     }
   }
-
-  for (int j0 = 0; j0 < n; j0 += W) {
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-      const int j = j0 + t;
-      if (j >= n) break;
-      // --- locate the pivot row j
-      int hit = -1;
-#pragma unroll
-      for (int r = 0; r < R; ++r) hit = (row[r] == j) ? r : hit;
-      const unsigned long long bal = __ballot(hit >= 0);
-      const int lp = __builtin_amdgcn_readfirstlane(__ffsll((long long)bal) - 1);
-      // --- pivot lane publishes its window (column order j..j+W-1) and rhs
-      if (lane == lp) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (hit == r) {
-            s_piv[0] = fast_rcp(w[r][t]);
-#pragma unroll
-            for (int k = 1; k < W; ++k) s_piv[k] = w[r][(t + k) % W];
-            s_piv[W] = b1[r];
-            s_piv[W + 1] = b2[r];
-          }
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double pv[W];
-#pragma unroll
-      for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
-      const double pb1 = s_piv[W], pb2 = s_piv[W + 1];
-      const double rinv = pv[0];                     // published as 1/U_jj
-      // --- U row j (1/U_jj, U_j,j+1..) straight from the published row, and the rhs;
-      // every lane stores entry lane % WP (same values, no branch; entry W is padding)
-      {
-        const int kk = lane % WP;
-        urows[(size_t)j * WP + kk] = s_piv[kk];
-        Y1[j] = pb1;
-        Y2[j] = pb2;
-      }
-      // --- eliminate column j, shift in column j+W (prefetched 2W steps ago)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double m = w[r][t] * rinv;
-#pragma unroll
-        for (int k = 1; k < W; ++k) w[r][(t + k) % W] = fma(-m, pv[k], w[r][(t + k) % W]);
-        b1[r] = fma(-m, pb1, b1[r]);
-        b2[r] = fma(-m, pb2, b2[r]);
-        const double v = fma(-mu, nm[r][t], nk[r][t]);
-        w[r][t] = ((vmask[r] >> t) & 1u) ? v : 0.0;
-      }
-      // --- retire the pivot row; its slot takes the next scheduled row (LDS reads only)
-      if (lane == lp) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (hit == r) {
-            const int nr = s_next[row[r]];
-            row[r] = nr;
-            vmask[r] = 0u;
-#pragma unroll
-            for (int k = 0; k < W; ++k) w[r][k] = 0.0;
-            if (nr >= 0) {
-              fc[r] = s_fcol[nr];
-              lc[r] = min(nr + bw, n - 1);
-              off[r] = s_poff[nr] - fc[r];
-              b1[r] = s_muhat[nr];
-              b2[r] = s_mx[nr];
-            } else {
-              fc[r] = 0x7fffffff; lc[r] = -1; off[r] = 0; b1[r] = 0.0; b2[r] = 0.0;
-            }
-          }
-        }
-      }
-      // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int c = j + 2 * W;
-        const bool valid = (c >= fc[r]) && (c <= lc[r]);
-        const int idx = valid ? off[r] + c : 0;
-        nk[r][t] = kprof[idx];
-        nm[r][t] = mprof[idx];
-        vmask[r] = (vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
-      }
+  // Y1/Y2 double as the entry values of the rhs (M uhat, M x): a row entering its
+  // slot during step j has index > j, and Y*[i] is only overwritten at pivot step i.
+  for (int j0 = 0, ch = 0; j0 < n; j0 += W, ++ch) {
+    const int levels = __builtin_amdgcn_readfirstlane((int)s_lev[ch]);
+    if (R == 1 || levels <= 1) {
+      factor_chunk<W, R, 1>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane);
+    } else {
+      factor_chunk<W, R, R>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane);
     }
   }
 }
@@ -330,7 +356,6 @@ __device__ void back_substitute(int n, const double* __restrict__ urows, int WP,
   for (int k = 0; k < W; ++k) z[k] = 0.0;
   const int nch = (n + CH - 1) / CH;
   double buf[PER];
-  // prime: chunk nch-1
   {
     const int base = (nch - 1) * CH * WP;
     const int lim = n * WP;
@@ -348,7 +373,6 @@ __device__ void back_substitute(int n, const double* __restrict__ urows, int WP,
   }
   __syncthreads();
   for (int c = nch - 1; c >= 0; --c) {
-    // issue the loads of chunk c-1
     if (c > 0) {
       const int base = (c - 1) * CH * WP;
 #pragma unroll
@@ -364,10 +388,10 @@ __device__ void back_substitute(int n, const double* __restrict__ urows, int WP,
         const int j = j0 + tt;
         if (j < n) {
           const double* ur = stg + tt * WP;
-          double s = Y[j];
+          double sum = Y[j];
 #pragma unroll
-          for (int k = W - 1; k >= 1; --k) s = fma(-ur[k], z[(tt + k) % W], s);
-          const double zj = s * ur[0];
+          for (int k = W - 1; k >= 1; --k) sum = fma(-ur[k], z[(tt + k) % W], sum);
+          const double zj = sum * ur[0];
           z[tt % W] = zj;
           Y[j] = zj;
         }
@@ -421,61 +445,71 @@ __global__ void k_prepare0d(GridDev g, BatchDev b) {
   }
 }
 
+// LDS layout (doubles first): X, Z1, Z2 (+ pad so Z1..Z2 can hold 4*npr mass Cholesky
+// coefficients), AV, STG[2W*WP], PIV[PIVN]; then PO (int), FC, NX, LEV (short).
+__host__ __device__ __forceinline__ int zpad(int n, int npr) { return (4 * npr > 2 * n) ? (4 * npr - 2 * n) : 0; }
+
 template <int W, int R>
 __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
   extern __shared__ double lds[];
   const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
-  const int WP = (W + 1) & ~1;
+  constexpr int WP = (W + 1) & ~1;
+  constexpr int PIVN = (W + 3) & ~1;
+  const int nchunk = (n + W - 1) / W;
   double* X = lds;
   double* Z1 = X + n;
   double* Z2 = Z1 + n;
-  double* MXv = Z2 + n;
-  double* MUH = MXv + n;
-  double* AV = MUH + n;                                    // M^-1 K x
-  double* WV = AV + n;                                     // growth vector w (mu = w.x)
-  double* UH = WV + n;                                     // uhat
-  double* PIV = UH + n;                                    // W + 2 (padded to 32)
-  double* LCH = PIV + 32;                                  // npr x 4 mass Cholesky rows
-  double* STG = LCH + 4 * g.npr;                           // 2W x WP back-substitution stage
-  int* FC = reinterpret_cast<int*>(STG + 2 * W * WP);
-  int* PO = FC + n;
-  int* NX = PO + (n + 1);
+  double* AV = Z2 + n + zpad(n, g.npr);                    // M^-1 K x
+  double* STG = AV + n;                                    // back-substitution stage
+  double* PIV = STG + 2 * W * WP;                          // published pivot row
+  int* PO = reinterpret_cast<int*>(PIV + PIVN);
+  short* FC = reinterpret_cast<short*>(PO + (n + 1));
+  short* NX = FC + n;
+  short* LEV = NX + n;
+  double* LCH = Z1;                                        // staged only when Z1/Z2 are free
 
-  const double* __restrict__ kprof = b.kprof + (size_t)inst * g.P;
+  double* kprof = b.kprof + (size_t)inst * g.P;           // holds K - mu_app M
   const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+  double mu_app = 0.0;
+  const double* __restrict__ UH = g.uhat;
+  const double* __restrict__ WV = b.vec + (size_t)inst * 4 * n;
   double* urows = b.urows + (size_t)inst * n * WP;
-  const double* wv_g = b.vec + (size_t)inst * 4 * n;
 
-  // ---------------- tables into LDS
+  // ---------------- tables into LDS, initial iterate (original (n,B) layout -> permuted)
   for (int i = lane; i < n; i += 64) {
-    FC[i] = g.first_col[i];
-    NX[i] = g.sched_next[i];
-    MUH[i] = g.muhat[i];
-    UH[i] = g.uhat[i];
-    WV[i] = wv_g[i];
+    FC[i] = (short)g.first_col[i];
+    NX[i] = (short)g.sched_next[i];
     X[i] = b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst];
   }
   for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
-  for (int e = lane; e < 4 * g.npr; e += 64) {
-    const int q = e >> 2, d = e & 3;
-    const double v = g.mchol[(size_t)(q * g.ns) * 4 + d];
-    LCH[e] = (d == 0) ? 1.0 / v : ((q - d >= 0) ? v : 0.0);
-  }
+  for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
   __syncthreads();
   {
-    double s = 0.0;
-    for (int i = lane; i < n; i += 64) s = fma(g.u[i], X[i], s);
-    const double ux = wave_sum(s);
+    double sacc = 0.0;
+    for (int i = lane; i < n; i += 64) sacc = fma(g.u[i], X[i], sacc);
+    const double ux = wave_sum(sacc);
     for (int i = lane; i < n; i += 64) X[i] = X[i] / ux;
   }
-  __syncthreads();
 
-  // ---------------- initial residual A = M^-1 K x
-  profile_matvec(g, kprof, FC, PO, X, AV, lane);
-  __syncthreads();
-  mass_solve(g.ns, g.npr, LCH, AV, lane);
-  __syncthreads();
-  double mu = dot_lds(WV, X, n, lane);
+  // residual A = M^-1 K x from scratch (uses Z1/Z2 as the coefficient stage)
+  auto true_residual = [&]() {
+    for (int e = lane; e < 4 * g.npr; e += 64) {
+      const int q = e >> 2, dd = e & 3;
+      const double v = g.mchol[(size_t)(q * g.ns) * 4 + dd];
+      LCH[e] = (dd == 0) ? 1.0 / v : ((q - dd >= 0) ? v : 0.0);
+    }
+    for (int i = lane; i < n; i += 64) AV[i] = 0.0;
+    __syncthreads();
+    flat_matvec(g.P, kprof, g.ent_rc, X, AV, lane);
+    __syncthreads();
+    mass_solve(g.ns, g.npr, LCH, AV, lane);
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) AV[i] = fma(mu_app, X[i], AV[i]);   // M^-1 (K - mu_app M + mu_app M) x
+    __syncthreads();
+  };
+
+  true_residual();
+  double mu = dot(WV, X, n, lane);
   const double r0 = proj_res_plain(AV, X, UH, n, mu, lane);
   double rcur = r0;
   double abs_err = r0;
@@ -489,13 +523,8 @@ __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
     if (!done && it >= b.max_iter) { status = 1; done = true; }
     if (done) {
       // recompute the residual from scratch; iterate on if the incremental value drifted
-      profile_matvec(g, kprof, FC, PO, X, MXv, lane);
-      __syncthreads();
-      mass_solve(g.ns, g.npr, LCH, MXv, lane);
-      __syncthreads();
-      for (int i = lane; i < n; i += 64) AV[i] = MXv[i];
-      __syncthreads();
-      mu = dot_lds(WV, X, n, lane);
+      true_residual();
+      mu = dot(WV, X, n, lane);
       const double rtrue = proj_res_plain(AV, X, UH, n, mu, lane);
       rcur = rtrue;
       if (status == 0 && rtrue > tol && it < b.max_iter && verify < 2) {
@@ -508,19 +537,34 @@ __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
       }
     }
     if (b.mode == 0) abs_err = rcur;
-    mu = dot_lds(WV, X, n, lane);
-    mass_matvec(g, mprof, FC, PO, X, MXv, lane);
+    mu = dot(WV, X, n, lane);
+    {
+      // shift the profile from K - mu_app M to K - mu M on the band of M
+      const double dmu = mu - mu_app;
+      const int nm = 7 * n;
+      for (int k = lane; k < nm; k += 64) {
+        const int i = k / 7, dd = k - 7 * (k / 7) - 3;
+        const int c = i + dd * g.ns;
+        if (c >= 0 && c < n) {
+          const int e = PO[i] + c - FC[i];
+          kprof[e] = fma(-dmu, mprof[e], kprof[e]);
+        }
+      }
+      mu_app = mu;
+    }
+    for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
+    mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, X, Z2, lane);
     __syncthreads();
-    factor_forward<W, R>(g, kprof, mu, FC, PO, NX, MUH, MXv, Z1, Z2, PIV, urows, WP, lane);
+    factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
     __syncthreads();
     back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
     // --- 2x2 system for (c, d)
-    const double s11 = dot_lds(UH, Z1, n, lane);
-    const double s12 = dot_lds(UH, Z2, n, lane);
-    const double ux = dot_lds(UH, X, n, lane);
-    const double wz1 = dot_lds(WV, Z1, n, lane);
-    const double wz2 = dot_lds(WV, Z2, n, lane);
+    const double s11 = dot(UH, Z1, n, lane);
+    const double s12 = dot(UH, Z2, n, lane);
+    const double ux = dot(UH, X, n, lane);
+    const double wz1 = dot(WV, Z1, n, lane);
+    const double wz2 = dot(WV, Z2, n, lane);
     double c, d;
     if (b.mode == 0) {
       c = ux / s11;
@@ -569,11 +613,18 @@ __global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
   }
 }
 
+template <int W>
+size_t newton_lds_bytes(int n, int npr) {
+  constexpr int WP = (W + 1) & ~1;
+  constexpr int PIVN = (W + 3) & ~1;
+  const int nchunk = (n + W - 1) / W;
+  return (size_t)(4 * n + zpad(n, npr) + 2 * W * WP + PIVN) * sizeof(double) + (size_t)(n + 1) * sizeof(int) +
+         (size_t)(2 * n + nchunk) * sizeof(short);
+}
+
 template <int W, int R>
 int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream) {
-  const int WP = (W + 1) & ~1;
-  const size_t lds = (size_t)(8 * g.n + 32 + 4 * g.npr + 2 * W * WP) * sizeof(double) +
-                     (size_t)(3 * g.n + 1) * sizeof(int);
+  const size_t lds = newton_lds_bytes<W>(g.n, g.npr);
   if (lds > 64 * 1024) return BZ_EINVAL;
   const long work = (long)b.B * g.P;
   const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
@@ -716,8 +767,9 @@ int bz_steady_solve_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, hipStream_
   g.nslots = gp->nslots;
   g.first_col = gp->first_col; g.prof_off = gp->prof_off; g.opprof = gp->opprof; g.slot_first = gp->slot_first;
   g.sched_next = gp->sched_next; g.muhat = gp->muhat; g.uhat = gp->uhat; g.u = gp->u; g.wsp = gp->wsp;
-  g.mchol = gp->mchol;
-  if (g.n != g.ns * g.npr || g.n < 8 || g.nop != g.S + 3 || g.nop > kMaxOps || g.S < 1) return BZ_EINVAL;
+  g.mchol = gp->mchol; g.chunk_levels = gp->chunk_levels; g.ent_rc = gp->ent_rc;
+  if (g.n != g.ns * g.npr || g.n < 8 || g.n > 32767 || g.nop != g.S + 3 || g.nop > kMaxOps || g.S < 1)
+    return BZ_EINVAL;
   BatchDev b;
   b.B = bp->B; b.n0 = bp->n0; b.Tg = bp->Tg; b.E = bp->E; b.ns_by_n0 = bp->ns_by_n0; b.x = bp->x;
   b.kprof = bp->kprof; b.urows = bp->urows; b.vec = bp->vec; b.iters = bp->iters; b.status = bp->status;

@@ -49,24 +49,31 @@ def plan_structure(mask_orig, npr, ns, window_fn):
     if W < 0:
         raise ValueError(f"upper bandwidth {bw} not supported by the 0D solver")
     # interval scheduling: row i occupies a slot from step first-2W to its pivot step i;
-    # a slot freed at step i_prev can take a row with first - 2W >= i_prev.
+    # a slot freed at step i_prev can take a row with first - 2W >= i_prev.  Rows are
+    # taken in order of their start and get the lowest free slot id, so slot ids >= 64
+    # (register level 1+, slot s -> lane s % 64, level s // 64) are used only where
+    # more than 64 rows are alive at once.
     start = first - PREFETCH_WINDOWS * W
     order = np.argsort(start, kind="stable")
-    free = []          # heap of (free_from_step, slot)
+    alive = []         # heap of (pivot_step, slot) of rows holding a slot
+    free_ids = []      # heap of free slot ids
     nslots_used = 0
     slot_of = np.full(n, -1, dtype=np.int64)
     slot_rows = []
     for i in order:
         s0 = int(start[i])
-        if free and free[0][0] <= s0:
-            _, s = heapq.heappop(free)
+        while alive and alive[0][0] <= s0:
+            _, s_rel = heapq.heappop(alive)
+            heapq.heappush(free_ids, s_rel)
+        if free_ids:
+            s = heapq.heappop(free_ids)
         else:
             s = nslots_used
             nslots_used += 1
             slot_rows.append([])
         slot_of[i] = s
         slot_rows[s].append(int(i))
-        heapq.heappush(free, (int(i), s))
+        heapq.heappush(alive, (int(i), s))
     R = max(1, math.ceil(nslots_used / 64))
     nslots = 64 * R
     slot_first = np.full(nslots, -1, dtype=np.int64)
@@ -75,6 +82,19 @@ def plan_structure(mask_orig, npr, ns, window_fn):
         slot_first[s] = rs[0]
         for a, b in zip(rs[:-1], rs[1:]):
             sched_next[a] = b
+    # register levels each W-step chunk must run: a row of level r needs its level
+    # executed from max(0, start) (first prefetch of its first column; earlier columns
+    # are zero and its window was zeroed at entry or loaded at step 0) to its pivot
+    # step, where it retires and its successor enters
+    nchunk = (n + W - 1) // W
+    levels = np.ones(nchunk, dtype=np.int64)
+    for s, rs in enumerate(slot_rows):
+        lev = s // 64 + 1
+        if lev == 1:
+            continue
+        for i in rs:
+            a = max(0, int(start[i]))
+            levels[a // W: i // W + 1] = np.maximum(levels[a // W: i // W + 1], lev)
     # invariants the kernel relies on
     for s, rs in enumerate(slot_rows):
         for a, b in zip(rs[:-1], rs[1:]):
@@ -82,10 +102,15 @@ def plan_structure(mask_orig, npr, ns, window_fn):
     assert (first <= np.arange(n)).all()
     # U stays in the band: no entry of the pattern above the band (no pivoting)
     assert ((cols - rows) <= bw).all()
+    # (row << 16) | column of every profile entry, for the flat matvec
+    ent_rc = np.concatenate([(np.uint32(i) << np.uint32(16)) | np.arange(first[i], last[i] + 1, dtype=np.uint32)
+                             for i in range(n)]).astype(np.uint32)
+    assert n < 32768
     return {
         "n": n, "bw": bw, "W": W, "R": R, "nslots": nslots, "perm": perm,
         "first_col": first, "prof_off": prof_off, "P": int(prof_off[-1]),
         "slot_first": slot_first, "sched_next": sched_next, "active_max": nslots_used,
+        "chunk_levels": levels, "ent_rc": ent_rc,
     }
 
 
@@ -114,6 +139,8 @@ class SolverTables:
         self.prof_off = i32(st["prof_off"])
         self.slot_first = i32(st["slot_first"])
         self.sched_next = i32(st["sched_next"])
+        self.chunk_levels = i32(st["chunk_levels"])
+        self.ent_rc = torch.from_numpy(st["ent_rc"].view(np.int32).copy()).to(dev)
         P = st["P"]
         self.P = P
         self.opprof = torch.empty((self.nop, P), dtype=torch.float64, device=dev)
@@ -158,7 +185,8 @@ class SolverTables:
             first_col=self.first_col.data_ptr(), prof_off=self.prof_off.data_ptr(),
             opprof=self.opprof.data_ptr(), slot_first=self.slot_first.data_ptr(),
             sched_next=self.sched_next.data_ptr(), muhat=self.muhat.data_ptr(), uhat=self.uhat.data_ptr(),
-            u=self.u.data_ptr(), wsp=self.wsp.data_ptr(), mchol=self.mchol.data_ptr())
+            u=self.u.data_ptr(), wsp=self.wsp.data_ptr(), mchol=self.mchol.data_ptr(),
+            chunk_levels=self.chunk_levels.data_ptr(), ent_rc=self.ent_rc.data_ptr())
 
     def scratch_bytes(self, B):
         return B * (self.P + self.n * self.WP + 4 * self.n) * 8

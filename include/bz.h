@@ -94,6 +94,8 @@ typedef struct {
   const double* u;        /* [n]   mass vector u                                */
   const double* wsp;      /* [nop-1][n] W_m = Op_m^T M^-1 u (zero row: operator not in the growth term) */
   const double* mchol;    /* [n][4] banded Cholesky rows of M                   */
+  const int* chunk_levels;    /* [ceil(n/W)] register levels alive per W-step chunk */
+  const unsigned* ent_rc;     /* [P] (row << 16) | column of every profile entry    */
 } bz_grid0d_t;
 
 /* Per-batch parameters, state and scratch (device pointers). */

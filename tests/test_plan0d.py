@@ -104,12 +104,14 @@ def emulate_factor(A, st):
     U = np.zeros((n, W))
     for j in range(n):
         t = j % W
-        hits = [s for s in range(nslot) if row[s] == j]
+        lev = st["chunk_levels"][j // W]
+        act = [s for s in range(nslot) if s // 64 < lev]      # levels the kernel runs in this chunk
+        hits = [s for s in act if row[s] == j]
         assert len(hits) == 1, (j, hits)
         sp = hits[0]
         pv = np.array([w[sp, (t + k) % W] for k in range(W)])
         U[j] = pv
-        for s in range(nslot):
+        for s in act:
             m = w[s, t] / pv[0]
             for k in range(1, W):
                 w[s, (t + k) % W] -= m * pv[k]
@@ -118,7 +120,7 @@ def emulate_factor(A, st):
         row[sp] = nr
         w[sp, :] = 0.0
         nk[sp, :] = 0.0
-        for s in range(nslot):
+        for s in act:
             nk[s, t] = val(row[s], j + PREFETCH_WINDOWS * W)
     return U
 
@@ -153,3 +155,4 @@ def test_plan_config2_fits_two_rows_per_lane():
     st = plan_structure(mask, g.num_p, g.num_sh, window_fn)
     assert st["bw"] == 10 and st["W"] == 11
     assert st["R"] == 2 and st["active_max"] <= 128
+    assert st["chunk_levels"].min() >= 1 and st["chunk_levels"].max() <= st["R"]
