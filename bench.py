@@ -9,8 +9,10 @@ excitation (11.5 eV) / ionization (15.76 eV) cross sections; n0 = 3.22e22 m^-3,
 Tg = 300 K, no e-e.
 
 One step = one batched solve of all 4096 instances from the Maxwellian initial
-guess to |r| <= 1e-12 |r0| (exact-Jacobian Newton) plus normalisation and QoIs,
-with operators and instance data resident in HBM.
+guess to |r| <= 1e-12 |r0| (exact-Jacobian Newton, at most 12 iterations) plus
+normalisation and QoIs, with operators and instance data resident in HBM.  Only
+converged instances count in solves/s: status 0, or a stop with |r| / |r0| < 1e-11
+(the FP64 roundoff floor of the residual).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
 """
@@ -30,9 +32,10 @@ XSEC = os.path.join(ROOT, "boltzmann_amd", "data", "eAr_synthetic_3proc.lxcat")
 B = 4096
 NR, LMAX, TE, EVMAX, Q = 128, 2, 4.0, 80.0, 5
 SEED = 20251202
-RTOL = 1e-10
+RTOL = 1e-12
 MAX_ITER = 12
-RATE_FLOOR = 1e-10
+RATE_FLOOR = 1e-9
+FLOOR_REL = 1e-11      # a stop with |r| / |r0| below this is at the FP64 roundoff floor
 
 
 def instances(nb, seed=SEED):
@@ -109,44 +112,57 @@ def plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0, rtol, max_iter):
     wb = (coef[:len(solver.C)].t() @ Wv)                        # (B, n)
     eye = torch.eye(n - 1, dtype=torch.float64, device=dev)
 
-    def residual(x):                                          # x (n, B)
-        Kx = torch.einsum("mb,mvu,ub->vb", coef, QTop, x)
-        mu = (wb * x.t()).sum(1)
-        return Kx - mu[None, :] * (QT @ x), mu
+    def solve_chunk(x0c, coefc, wbc):
+        Bc = x0c.shape[1]
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    x = x0 / (u @ x0)[None, :]
-    r, mu = residual(x)
-    rn = torch.linalg.norm(r, dim=0)
-    r0 = rn.clone()
-    active = torch.ones(B, dtype=torch.bool, device=dev)
-    iters = torch.zeros(B, dtype=torch.int32, device=dev)
-    for it in range(max_iter):
-        active = active & (rn > rtol * r0)
-        if not bool(active.any()):
-            break
-        J = torch.einsum("mb,mij->bij", coef, QTopQ) - mu[:, None, None] * eye[None]
-        J = J - torch.einsum("ib,bj->bij", QT @ x, wb @ Q)
-        dz = torch.linalg.solve(J, -r.t()).t()
-        p = Q @ dz
-        alpha = torch.ones(B, dtype=torch.float64, device=dev)
-        for _ in range(60):
-            rt, _m = residual(x + alpha[None, :] * p)
-            rtn = torch.linalg.norm(rt, dim=0)
-            bad = (0.5 * rtn ** 2 > 0.5 * rn ** 2 - 1e-4 * alpha * rn ** 2) & active
-            if not bool(bad.any()):
-                break
-            alpha = torch.where(bad, alpha * 0.5, alpha)
-        step = torch.where(active, alpha, torch.zeros_like(alpha))
-        x = x + step[None, :] * p
-        iters += active.to(torch.int32)
+        def residual(x):                                      # x (n, Bc)
+            Kx = torch.einsum("mb,mvu,ub->vb", coefc, QTop, x)
+            mu = (wbc * x.t()).sum(1)
+            return Kx - mu[None, :] * (QT @ x), mu
+
+        x = x0c / (u @ x0c)[None, :]
         r, mu = residual(x)
         rn = torch.linalg.norm(r, dim=0)
+        r0 = rn.clone()
+        active = torch.ones(Bc, dtype=torch.bool, device=dev)
+        iters = torch.zeros(Bc, dtype=torch.int32, device=dev)
+        for it in range(max_iter):
+            active = active & (rn > rtol * r0)
+            if not bool(active.any()):
+                break
+            J = torch.einsum("mb,mij->bij", coefc, QTopQ) - mu[:, None, None] * eye[None]
+            J = J - torch.einsum("ib,bj->bij", QT @ x, wbc @ Q)
+            dz = torch.linalg.solve(J, -r.t()).t()
+            p = Q @ dz
+            alpha = torch.ones(Bc, dtype=torch.float64, device=dev)
+            for _ in range(60):
+                rt, _m = residual(x + alpha[None, :] * p)
+                rtn = torch.linalg.norm(rt, dim=0)
+                bad = (0.5 * rtn ** 2 > 0.5 * rn ** 2 - 1e-4 * alpha * rn ** 2) & active
+                if not bool(bad.any()):
+                    break
+                alpha = torch.where(bad, alpha * 0.5, alpha)
+            step = torch.where(active, alpha, torch.zeros_like(alpha))
+            x = x + step[None, :] * p
+            iters += active.to(torch.int32)
+            r, mu = residual(x)
+            rn = torch.linalg.norm(r, dim=0)
+        return int((rn <= rtol * r0).sum()), iters
+
+    # hipBLAS's batched LU (getrfBatched) reports an allocation failure above 64
+    # systems of 386^2 on this stack: the plain form runs the batch in chunks of 64
+    CHK = 64
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    conv = 0
+    its = []
+    for a in range(0, B, CHK):
+        c, it = solve_chunk(x0[:, a:a + CHK], coef[:, a:a + CHK], wb[a:a + CHK])
+        conv += c
+        its.append(it)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    conv = int((rn <= rtol * r0).sum())
-    return conv / dt, float(iters.double().mean()), conv, dt
+    return conv / dt, float(torch.cat(its).double().mean()), conv, dt
 
 
 def accuracy_check(h_gpu, q_gpu, idx):
@@ -182,7 +198,19 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU form")
     ap.add_argument("--mode", type=int, default=1, help="1: exact Jacobian, 0: reference Jacobian")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device once (HBM byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (instances shard "
               f"trivially across GPUs; this build measures one GPU's share)", file=sys.stderr)
@@ -249,7 +277,8 @@ def main():
     achieved = flops_launch / (kernel_ms.mean() * 1e-3) / 1e12
     peak = 78.6
     dense_ref = (g.n - 1) ** 3 + 2 * (g.n - 1) ** 2      # reference model of the dense Jacobian solve
-    conv = int((status == 0).sum())
+    rel = (sc["res_abs"] / sc["res0"]).cpu().numpy()
+    conv = int(((status == 0) | (rel < FLOOR_REL)).sum())
     value = conv * args.steps / elapsed
 
     line = {

@@ -449,8 +449,10 @@ __global__ void k_prepare0d(GridDev g, BatchDev b) {
 // coefficients), AV, STG[2W*WP], PIV[PIVN]; then PO (int), FC, NX, LEV (short).
 __host__ __device__ __forceinline__ int zpad(int n, int npr) { return (4 * npr > 2 * n) ? (4 * npr - 2 * n) : 0; }
 
+// Two waves per SIMD (register budget 256 incl. spills) measured faster than one
+// wave holding every value in registers.
 template <int W, int R>
-__global__ __launch_bounds__(64) void k_newton0d(GridDev g, BatchDev b) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_newton0d(GridDev g, BatchDev b) {
   extern __shared__ double lds[];
   const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
   constexpr int WP = (W + 1) & ~1;

@@ -28,6 +28,13 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
 
 
+def _converged(info, floor=1e-12):
+    """Residual at the FP64 roundoff floor: status 0, or a stop (line search or the
+    verification of the residual from scratch) with |r| / |r0| below the floor."""
+    rel = info["res_abs"] / info["res0"]
+    return (info["status"] == 0) | (rel < floor)
+
+
 def _gpu_solve(solver, prm, mode, rtol=1e-13, max_iter=60):
     B = prm["n0"].shape[0]
     n0, Tg, E, ns = _dev(prm["n0"]), _dev(prm["Tg"]), _dev(prm["E"]), _dev(prm["ns_by_n0"])
@@ -56,8 +63,8 @@ def test_setup_tables_small(small):
 
 def test_exact_newton_matches_oracle_small(small):
     ops, solver, prm, h_ref, q_ref = small
-    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-13)
-    assert (info["status"] == 0).all(), info
+    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-14)
+    assert _converged(info).all(), info
     assert rel_err(h, h_ref) < QOI_TOL
     errs = qoi_errors(q, q_ref)
     assert max(errs.values()) < QOI_TOL, errs
@@ -105,8 +112,8 @@ def test_config2_size_parity():
     ops, g, procs, solver = _setup(128, 2)
     prm = instance_params(6, seed=11)
     h_ref, q_ref, _ = s0.steady_state_solve_each(ops, prm, ops.initial(6), 0.0, 1e-15, 30, jacobian="exact")
-    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-13)
-    assert (info["status"] == 0).all(), info
+    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-14)
+    assert _converged(info).all(), info
     assert rel_err(h, h_ref) < QOI_TOL
     errs = qoi_errors(q, q_ref)
     assert max(errs.values()) < QOI_TOL, errs
