@@ -36,6 +36,20 @@ RTOL = 1e-12
 MAX_ITER = 12
 RATE_FLOOR = 1e-9
 FLOOR_REL = 1e-11      # a stop with |r| / |r0| below this is at the FP64 roundoff floor
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_newton0d_b4096_n387_hbm_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per solver launch (k_prepare0d + k_newton0d) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this workload, each scaled by the
+    counter's reading for a 1 GiB copy (the file states commands and factors)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def instances(nb, seed=SEED):
@@ -296,9 +310,11 @@ def main():
                          "converged": int((status == 0).sum()), "of": B},
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_newton0d<11,2>", "kernel_ms": float(kernel_ms.mean()),
-                     "note": "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s"},
+                     "frac": achieved / peak, "traffic": measured_traffic(),
+                     "kernel": "k_prepare0d + k_newton0d<11,2> (one bz_steady_solve_0d call)",
+                     "kernel_ms": float(kernel_ms.mean()),
+                     "note": "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
+                             "achieved = algorithmic flops of the profile-LU Newton iterations run / launch time"},
         "reference_equivalent_tflops": float(dense_ref * iters.sum() / (kernel_ms.mean() * 1e-3) / 1e12),
     }
     if not args.no_check:
