@@ -144,6 +144,50 @@ int bz_gemv(const double* A, int m, int k, int trans, const double* x, double al
 int bz_permute(const double* in, int n, int ns, int npr, int inverse, double* out, hipStream_t stream);
 int bz_mchol_perm(const double* lrev, int ns, int npr, double* out, hipStream_t stream);
 
+/* ------------------------------------------------------------------ 1D3V batched sweeps */
+
+/* Batched bidiagonal solve, one system per row: diag (bsz, N), subdiag (bsz, N-1),
+ * b, out (bsz, N); lower != 0: forward substitution with subdiag below the diagonal,
+ * else backward substitution with subdiag above it.
+ * Replaces rawkernel.lower_bidiagonal_batched / upper_bidiagonal_batched and
+ * bidiagonal_solve_batched (rawkernel.py:6-46, 389-413). */
+int bz_bidiag_solve_batched(const double* diag, const double* subdiag, const double* b, double* out, int bsz, int N,
+                            int lower, hipStream_t stream);
+
+/* Batched banded triangular solve: diags (bsz, ndiags, N), diagonal d of row i at
+ * [d, i] (coupling row i to i-d if lower, i+d otherwise), ndiags <= 8.
+ * Replaces lower/upper_banded_tri_batched and banded_tri_solve_batched
+ * (rawkernel.py:48-109, 415-439). */
+int bz_banded_tri_solve_batched(const double* diags, const double* b, double* out, int bsz, int ndiags, int N,
+                                int lower, hipStream_t stream);
+
+/* Batched banded triangular product out = T b with the same layout.
+ * Replaces lower/upper_banded_tri_batched_mv and banded_tri_mv_batched
+ * (rawkernel.py:111-166, 441-465). */
+int bz_banded_tri_mv_batched(const double* diags, const double* b, double* out, int bsz, int ndiags, int N, int lower,
+                             hipStream_t stream);
+
+/* Backward-Euler upwind advection sweeps with shared operators D_LtoR, D_RtoL
+ * (ndiags, N), ndiags <= 8, direction per system from the sign of the speed.
+ * vr: b, out (Nx*Nvt, Nr), speed -Ex[ix] cos_vt[ivt];  vt: b, out (Nx*Nr, Nvt), speed
+ * Ex[ix] / vr[ivr];  x: b, out (Nr*Nvt, Nx), speed vr[ivr] cos_vt[ivt].
+ * Replace adv_vr_BE_solve / adv_vt_BE_solve / adv_x_BE_solve and their wrappers
+ * (rawkernel.py:169-375, 468-503). */
+int bz_adv_vr_be(const double* D_LtoR, const double* D_RtoL, const double* Ex, const double* cos_vt, double dt,
+                 const double* b, double* out, int ndiags, int Nr, int Nvt, int Nx, hipStream_t stream);
+int bz_adv_vt_be(const double* D_LtoR, const double* D_RtoL, const double* Ex, const double* vr, double dt,
+                 const double* b, double* out, int ndiags, int Nr, int Nvt, int Nx, hipStream_t stream);
+int bz_adv_x_be(const double* D_LtoR, const double* D_RtoL, const double* vr, const double* cos_vt, double dt,
+                const double* b, double* out, int ndiags, int Nr, int Nvt, int Nx, hipStream_t stream);
+
+/* x = U^-1 b for a block upper-triangular BSR matrix with unit diagonal blocks (diagonal
+ * and lower blocks skipped): data (nnzb, bsz, bsz), indptr (nbr+1), indices (nnzb);
+ * b, out column-major (nbr*bsz, nrhs); bsz <= 256.
+ * Replaces spsm_bsr_upper_tri_solve (BESolver/cuda/spsm_bsr_upper_tri_solve.cu:2-58)
+ * as launched by SpSMSolver.solve (cusparse_spsolve.py:366-372). */
+int bz_bsr_upper_unit_solve(const double* data, const int* indptr, const int* indices, int nbr, int nrhs, int bsz,
+                            const double* b, double* out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
