@@ -41,7 +41,7 @@ class Batch0D(ctypes.Structure):
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;
 # every entry point returns int
 _SIGS = {}
-_CT = {"p": c_void_p, "i": c_int, "d": c_double, "s": c_void_p}
+_CT = {"p": c_void_p, "i": c_int, "d": c_double, "s": c_void_p, "l": ctypes.c_long}
 
 
 HEADER = os.path.abspath(os.path.join(HERE, "..", "include", "bz.h"))
@@ -65,6 +65,8 @@ def header_signatures(path=HEADER):
                     code += "p"
                 elif p.startswith("double"):
                     code += "d"
+                elif p.startswith("long"):
+                    code += "l"
                 elif p.startswith("int"):
                     code += "i"
                 else:

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbz.so")
 INCLUDE = os.path.abspath(os.path.join(HERE, "..", "include"))
 
-SOURCES = ["bz_assembly.hip", "bz_solver0d.hip", "bz_banded.hip", "bz_ee.hip"]
+SOURCES = ["bz_assembly.hip", "bz_solver0d.hip", "bz_banded.hip", "bz_krylov.hip", "bz_ee.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

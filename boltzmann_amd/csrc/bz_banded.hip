@@ -19,7 +19,7 @@ namespace {
 
 constexpr int TS = 64;   // systems per workgroup (one wavefront)
 
-enum Kind { K_BIDIAG = 0, K_BANDED = 1, K_ADV_VR = 2, K_ADV_VT = 3, K_ADV_X = 4 };
+enum Kind { K_BIDIAG = 0, K_BANDED = 1, K_ADV_VR = 2, K_ADV_VT = 3, K_ADV_X = 4, K_XSTEP = 5 };
 
 template <int ND>
 struct ChunkCols {
@@ -42,6 +42,7 @@ struct SweepArgs {
   double* out;
   int bsz, N, lower;
   int Nr, Nvt, Nx;
+  const int* lower_of;   // K_XSTEP: direction of system s is lower_of[s % Nvt]
 };
 
 // advection speed and direction of system s (rawkernel.py:183-192, 249-254, 316-322)
@@ -75,7 +76,8 @@ template <int KIND, int ND>
 __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
   constexpr int CH = ChunkCols<ND>::CH;
   constexpr int ST = ChunkCols<ND>::ST;
-  constexpr int NC = (KIND == K_BANDED) ? ND : (KIND == K_BIDIAG ? 2 : 0);   // staged coefficient tiles
+  constexpr bool BIDIAG = (KIND == K_BIDIAG || KIND == K_XSTEP);
+  constexpr int NC = (KIND == K_BANDED) ? ND : (BIDIAG ? 2 : 0);   // staged coefficient tiles
   __shared__ double tb[TS * ST];
   __shared__ double tc[(NC > 0 ? NC : 1) * TS * ST];
   __shared__ int fwd_s[TS];
@@ -87,8 +89,9 @@ __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
   const int N = a.N;
 
   double av = 0.0;
-  bool fwd = !(KIND == K_BIDIAG || KIND == K_BANDED) ? true : (a.lower != 0);
-  if (KIND >= K_ADV_VR && active) adv_speed<KIND>(a, s, av, fwd);
+  bool fwd = !(BIDIAG || KIND == K_BANDED) ? true : (a.lower != 0);
+  if (KIND >= K_ADV_VR && KIND <= K_ADV_X && active) adv_speed<KIND>(a, s, av, fwd);
+  if (KIND == K_XSTEP && active) fwd = a.lower_of[s % a.Nvt] != 0;
   fwd_s[lane] = fwd ? 1 : 0;
   __syncthreads();
 
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
         if (KIND == K_BANDED) {
 #pragma unroll
           for (int d = 0; d < ND; ++d) tc[(d * TS + r) * ST + o] = a.diags[((size_t)sr * ND + d) * N + c];
-        } else if (KIND == K_BIDIAG) {
+        } else if (BIDIAG) {
           tc[r * ST + o] = a.diag[(size_t)sr * N + c];
           // coupling of row c to its upwind neighbour: subdiag[c-1] (lower) or subdiag[c] (upper)
           const int sc = fwd_s[r] ? c - 1 : c;
@@ -124,9 +127,10 @@ __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
       for (int q = 0; q < len; ++q, ++p) {
         const int o = fwd ? q : (len - 1 - q);                 // tile column
         const int c = fwd ? (k * CH + o) : (N - k * CH - len + o);   // matrix row
-        const double bi = tb[lane * ST + o];
+        // the x-step zeroes the inflow value of every system before its sweep
+        const double bi = (KIND == K_XSTEP && p == 0) ? 0.0 : tb[lane * ST + o];
         double val;
-        if (KIND == K_BIDIAG) {
+        if (BIDIAG) {
           const double sv = (p >= 1) ? tc[(TS + lane) * ST + o] * y[0] : 0.0;
           val = (bi - sv) / tc[lane * ST + o];
         } else if (KIND == K_BANDED) {
@@ -231,9 +235,37 @@ __global__ void k_bsr_upper_unit(const double* __restrict__ data, const int* __r
   }
 }
 
+// out[r, l, x] = sum_i P[l, i] in[r, i, x]: P (m, k) applied to every (k, nx) slab of
+// in (nr, k, nx); consecutive threads take consecutive x.
+__global__ void k_slab_left_mult(const double* __restrict__ P, int m, int k, const double* __restrict__ in, int nr,
+                                 int nx, double* __restrict__ out) {
+  const size_t total = (size_t)nr * m * nx;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % nx);
+    const size_t rl = e / nx;
+    const int l = (int)(rl % m), r = (int)(rl / m);
+    const double* src = in + (size_t)r * k * nx + x;
+    const double* pl = P + (size_t)l * k;
+    double acc = 0.0;
+    for (int i = 0; i < k; ++i) acc += pl[i] * src[(size_t)i * nx];
+    out[e] = acc;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int bz_slab_left_mult(const double* P, int m, int k, const double* in, int nr, int nx, double* out,
+                      hipStream_t stream) {
+  if (m < 1 || k < 1 || nr < 0 || nx < 0 || !P || !in || !out) return BZ_EINVAL;
+  const size_t total = (size_t)nr * m * nx;
+  if (total == 0) return BZ_OK;
+  const int blocks = (int)std::min<size_t>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_slab_left_mult, dim3(blocks), dim3(256), 0, stream, P, m, k, in, nr, nx, out);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
 
 int bz_bidiag_solve_batched(const double* diag, const double* subdiag, const double* b, double* out, int bsz, int N,
                             int lower, hipStream_t stream) {
@@ -242,6 +274,15 @@ int bz_bidiag_solve_batched(const double* diag, const double* subdiag, const dou
   SweepArgs a{};
   a.diag = diag; a.sdiag = subdiag; a.b = b; a.out = out; a.bsz = bsz; a.N = N; a.lower = lower;
   return launch_sweep<K_BIDIAG>(a, 2, stream);
+}
+
+int bz_xstep_bidiag_solve(const double* diag, const double* subdiag, const int* lower_of, const double* b,
+                          double* out, int Nr, int Nvt, int Nx, hipStream_t stream) {
+  if (Nr < 1 || Nvt < 1 || Nx < 2 || !diag || !subdiag || !lower_of || !b || !out) return BZ_EINVAL;
+  SweepArgs a{};
+  a.diag = diag; a.sdiag = subdiag; a.lower_of = lower_of; a.b = b; a.out = out;
+  a.bsz = Nr * Nvt; a.N = Nx; a.Nr = Nr; a.Nvt = Nvt; a.Nx = Nx;
+  return launch_sweep<K_XSTEP>(a, 2, stream);
 }
 
 int bz_banded_tri_solve_batched(const double* diags, const double* b, double* out, int bsz, int ndiags, int N,

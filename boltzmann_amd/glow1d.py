@@ -1,0 +1,224 @@
+"""1D3V velocity-space pieces of the glow-discharge Boltzmann step, on the MI355X.
+
+Mirrors the v-space side of glow1d_boltzmann (BESolver/python/glowdischarge_boltzmann_1d.py):
+  * ordinates theta_j and weights (:180-185), po2sh / psh2o (:330-334),
+    ords_to_sph / sph_to_ords (:1032-1048)        -> bz_slab_left_mult
+  * the x-advection half step: adv_x = M^-1 (vth tau / L) [int v^3 B_p B_k] (:369-472),
+    its eigen decomposition, the upwind backward-Euler shift operators
+    (:718-790, USE_BE_UPW_FD) and step_bte_x (:1220-1287)
+                                                   -> rocBLAS GEMMs with Q^-1, Q and one
+                                                      bz_xstep_bidiag_solve launch
+The state is v (Nr*Nvt, Nx): radial B-spline coefficient r, ordinate j, point x, with x
+fastest (the reference's layout).  Host work is set-up only (the eigen decomposition
+of one Nr x Nr matrix, as in the reference); every per-step operation runs on the GPU.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def ordinates(Nvt):
+    """glowdischarge_boltzmann_1d.py:180-185: Nvt/2 Gauss-Legendre points on each half
+    of cos(theta) in [-1, 1]; returns (theta, weights)."""
+    if Nvt % 2:
+        raise ValueError("Nvt must be even")
+    gx, gw = np.polynomial.legendre.leggauss(Nvt // 2)
+    mu = np.append(0.5 * gx - 0.5, 0.5 * gx + 0.5)
+    return np.arccos(mu), np.append(0.5 * gw, 0.5 * gw)
+
+
+def sph_ordinate_ops(lmax, theta, w):
+    """(po2sh (ns, Nvt), psh2o (Nvt, ns)) with Y_l0(theta) real orthonormal (:330-334)."""
+    mu = np.cos(theta)
+    Y = np.zeros((lmax + 1, len(theta)))
+    for l in range(lmax + 1):
+        c = np.zeros(l + 1)
+        c[l] = 1.0
+        Y[l] = math.sqrt((2 * l + 1) / (4 * math.pi)) * np.polynomial.legendre.legval(mu, c)
+    return Y * w[None, :] * 2 * math.pi, np.ascontiguousarray(Y.T)
+
+
+def slab_left_mult(P, x, nr, nx):
+    """out[r] = P @ x[r] for the (k, nx) slabs of x (nr*k, nx)."""
+    m, k = P.shape
+    assert x.shape == (nr * k, nx) and x.dtype == torch.float64
+    out = torch.empty((nr * m, nx), dtype=torch.float64, device=x.device)
+    _lib.call("bz_slab_left_mult", P.contiguous(), m, k, x.contiguous(), nr, nx, out)
+    return out
+
+
+class Ordinates:
+    """Ordinate <-> spherical-harmonic transforms on the device."""
+
+    def __init__(self, num_p, lmax, Nvt, device):
+        self.num_p, self.num_sh, self.Nvt = num_p, lmax + 1, Nvt
+        self.theta, self.w = ordinates(Nvt)
+        po2sh, psh2o = sph_ordinate_ops(lmax, self.theta, self.w)
+        self.po2sh = torch.from_numpy(po2sh).to(device)
+        self.psh2o = torch.from_numpy(psh2o).to(device)
+
+    def ords_to_sph(self, x):
+        """(num_p*Nvt, Nx) -> (num_p*num_sh, Nx) (:1032-1039)."""
+        return slab_left_mult(self.po2sh, x, self.num_p, x.shape[1])
+
+    def sph_to_ords(self, x):
+        """(num_p*num_sh, Nx) -> (num_p*Nvt, Nx) (:1041-1048)."""
+        return slab_left_mult(self.psh2o, x, self.num_p, x.shape[1])
+
+
+def upwind_first_order(x):
+    """Two-point one-sided first derivatives on the points x: D_L (backward differences,
+    first row zero) and D_R (forward differences, last row zero)."""
+    N = len(x)
+    h = np.diff(x)
+    dl_d = np.zeros(N)
+    dl_s = np.zeros(N - 1)
+    dr_d = np.zeros(N)
+    dr_s = np.zeros(N - 1)
+    dl_d[1:] = 1.0 / h
+    dl_s[:] = -1.0 / h            # D_L[i, i-1], i = 1..N-1
+    dr_d[:-1] = -1.0 / h
+    dr_s[:] = 1.0 / h             # D_R[i, i+1], i = 0..N-2
+    return (dl_d, dl_s), (dr_d, dr_s)
+
+
+class XAdvection:
+    """initialize_bte_adv_x (upwind finite differences) + step_bte_x on the device.
+
+    vgrid: VGrid of the radial basis; xgrid: the Nx spatial points; Nvt ordinates;
+    scale = vth * tau / L; dt: the half-step size the shift operators are built for."""
+
+    def __init__(self, vgrid, xgrid, Nvt, scale, dt):
+        dev = vgrid.dev
+        self.dev = dev
+        self.dt = dt
+        Nr = vgrid.npr
+        self.Nr, self.Nvt, self.Nx = Nr, Nvt, len(xgrid)
+        self.theta, _ = ordinates(Nvt)
+        # adv_x = M^-1 scale S3 (radial), exact Gauss rule (the same rule for M and S3)
+        S3 = vgrid.radial_bilinear(vgrid.sp_order + 8, 3, 0, 0).cpu().numpy()
+        M = vgrid.radial_bilinear(vgrid.sp_order + 8, 2, 0, 0).cpu().numpy()
+        adv_x = np.linalg.solve(M, scale * S3)
+        d, Q = np.linalg.eig(adv_x)
+        if np.abs(np.imag(d)).max() > 0 or np.abs(np.imag(Q)).max() > 0:
+            raise ValueError("x-advection operator has complex eigenvalues")
+        d, Q = np.real(d), np.real(Q)
+        self.adv_x = adv_x
+        self.d = d
+        self.Q = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
+        self.Qinv = torch.from_numpy(np.ascontiguousarray(np.linalg.inv(Q))).to(dev)
+        (dl_d, dl_s), (dr_d, dr_s) = upwind_first_order(np.asarray(xgrid, dtype=np.float64))
+        lower = self.theta <= 0.5 * np.pi
+        a = dt * d[:, None] * np.cos(self.theta)[None, :]             # (Nr, Nvt)
+        diag = np.where(lower[None, :, None], 1.0 + a[:, :, None] * dl_d[None, None, :],
+                        1.0 + a[:, :, None] * dr_d[None, None, :])
+        sdiag = np.where(lower[None, :, None], a[:, :, None] * dl_s[None, None, :],
+                         a[:, :, None] * dr_s[None, None, :])
+        self.diag = torch.from_numpy(np.ascontiguousarray(diag.reshape(Nr * Nvt, -1))).to(dev)
+        self.sdiag = torch.from_numpy(np.ascontiguousarray(sdiag.reshape(Nr * Nvt, -1))).to(dev)
+        self.lower_of = torch.from_numpy(lower.astype(np.int32)).to(dev)
+
+    def step(self, v, dt=None):
+        """step_bte_x (:1220-1287): v (Nr*Nvt, Nx) -> advected v."""
+        return self._step(v, dt)
+
+    def _step(self, v, dt):
+        if dt is not None and dt != self.dt:
+            raise AssertionError("x-advection operators were set up for another dt")
+        Nr, Nvt, Nx = self.Nr, self.Nvt, self.Nx
+        z = (self.Qinv @ v.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx).contiguous()
+        y = torch.empty_like(z)
+        _lib.call("bz_xstep_bidiag_solve", self.diag, self.sdiag, self.lower_of, z, y, Nr, Nvt, Nx)
+        return (self.Q @ y.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx)
+
+
+def default_pc_fields(num_pc_evals=30):
+    """Preconditioner field values of the reference set-up (:2087-2092): +-logspace(1e2, 6e5)."""
+    ep = np.logspace(np.log10(1e2), np.log10(6e5), num_pc_evals // 2, base=10)
+    return np.append(-np.flip(ep), ep)
+
+
+class VStep:
+    """Backward-Euler v-space step of the glow case (step_bte_v, ts_type "BE",
+    glowdischarge_boltzmann_1d.py:1476-1592) on the device.
+
+    C_en, C_gT, A_v: (dof, dof) device operators already premultiplied by M^-1 (the
+    reference's op_col_en, op_col_gT, op_adv_v).  For a spatially varying field the
+    system (I - dt tau (n0 np0 (C_en + Tg C_gT) + E(x) A_v)) y = u over all points is
+    solved by right-preconditioned restarted GMRES with the per-point inverse taken
+    from the table built at E-bin values (:2087-2112) and chosen by the nearest bin
+    (:1429-1474); a uniform field is solved directly (:1561-1572)."""
+
+    def __init__(self, C_en, C_gT, A_v, tau, n0, np0, Tg, dt, ordinates=None, Evals=None, rtol=1e-10, atol=1e-20,
+                 gmres_rsrt=30):
+        dev = C_en.device
+        self.dev = dev
+        self.dof = C_en.shape[0]
+        self.tau, self.dt = float(tau), float(dt)
+        self.K0 = (n0 * np0 * (C_en + Tg * C_gT)).contiguous()
+        self.A = A_v.contiguous()
+        self.ords = ordinates
+        self.rtol, self.atol, self.rsrt = rtol, atol, int(gmres_rsrt)
+        Ev = default_pc_fields() if Evals is None else np.asarray(Evals, dtype=np.float64)
+        self.Evals = torch.from_numpy(Ev).to(dev)
+        I = torch.eye(self.dof, dtype=torch.float64, device=dev)
+        c = self.dt * self.tau
+        mats = [I - c * Ev[0] * self.A - c * self.K0]
+        for i in range(1, len(Ev)):
+            mats.append(I - c * 0.5 * (Ev[i - 1] + Ev[i]) * self.A - c * self.K0)
+        mats.append(I - c * Ev[-1] * self.A - c * self.K0)
+        self.PmatE = torch.linalg.inv(torch.stack(mats)).contiguous()     # (len(Ev)+1, dof, dof), set-up only
+        self.last_iters = 0
+
+    def _lmat(self, E):
+        nx = E.shape[0]
+        dof = self.dof
+        c = self.dt * self.tau
+
+        def mv(x):
+            X = x.reshape(dof, nx)
+            Kx = self.K0 @ X
+            Ax = self.A @ X
+            y = torch.empty_like(X)
+            _lib.call("bz_vstep_combine", X.contiguous(), Kx, Ax, E, c, dof, nx, y)
+            return y.reshape(-1)
+        return mv
+
+    def _pc(self, E):
+        nx = E.shape[0]
+        bins = torch.empty(nx, dtype=torch.int32, device=self.dev)
+        _lib.call("bz_nearest_bin", E, nx, self.Evals, self.Evals.shape[0], bins)
+        dof = self.dof
+
+        def mv(x):
+            y = torch.empty((dof, nx), dtype=torch.float64, device=self.dev)
+            _lib.call("bz_binned_block_apply", self.PmatE, dof, bins, x.reshape(dof, nx).contiguous(), nx, y)
+            return y.reshape(-1)
+        return mv
+
+    def solve_sph(self, u, E):
+        """u (dof, Nx) in the harmonic basis, E (Nx,) device -> y (dof, Nx)."""
+        from .krylov import gmres_pcr
+        E = E.contiguous().to(torch.float64)
+        nx = E.shape[0]
+        if bool((E == E[0]).all()):
+            L = torch.eye(self.dof, dtype=torch.float64, device=self.dev) - self.dt * self.tau * (self.K0 + E[0] * self.A)
+            return torch.linalg.inv(L) @ u
+        counter = [0]
+
+        def cb(_):
+            counter[0] += 1
+        y, status = gmres_pcr(self._lmat(E), u.reshape(-1), x0=u.reshape(-1), atol=self.atol, rtol=self.rtol,
+                              maxiter=self.rsrt * 50, restart=self.rsrt, M=self._pc(E), callback=cb)
+        self.last_iters = counter[0]
+        if status != 0:
+            raise RuntimeError(f"GMRES did not converge in {status} iterations")
+        return y.reshape(self.dof, nx)
+
+    def step(self, v, E):
+        """v (num_p*Nvt, Nx) ordinates -> ordinates after one BE v-step."""
+        u = self.ords.ords_to_sph(v)
+        return self.ords.sph_to_ords(self.solve_sph(u, E).contiguous())

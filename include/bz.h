@@ -154,6 +154,22 @@ int bz_mchol_perm(const double* lrev, int ns, int npr, double* out, hipStream_t 
 int bz_bidiag_solve_batched(const double* diag, const double* subdiag, const double* b, double* out, int bsz, int N,
                             int lower, hipStream_t stream);
 
+/* x-advection sweep of the 1D3V x-step: systems s = i*Nvt + j (i: eigenmode of the radial
+ * x-advection operator, j: ordinate) of length Nx with bidiagonal (I + dt d_i cos(theta_j) D)
+ * stored as diag (Nr*Nvt, Nx), subdiag (Nr*Nvt, Nx-1); lower_of[j] != 0 selects left-to-right
+ * upwinding (D_L, coupling to x-1), else right to left (D_R, coupling to x+1); the inflow value
+ * of every system is taken as zero (the boundary condition).
+ * Replaces the two masked bidiagonal_solve_batched calls and the inflow zeroing of
+ * glow1d_boltzmann.step_bte_x (glowdischarge_boltzmann_1d.py:1237-1256). */
+int bz_xstep_bidiag_solve(const double* diag, const double* subdiag, const int* lower_of, const double* b,
+                          double* out, int Nr, int Nvt, int Nx, hipStream_t stream);
+
+/* out[r, l, x] = sum_i P[l, i] in[r, i, x] for r < nr: P (m, k), in (nr, k, nx), out (nr, m, nx).
+ * With P = po2sh (ns, Nvt) or psh2o (Nvt, ns) this is ords_to_sph / sph_to_ords
+ * (glowdischarge_boltzmann_1d.py:1032-1048; operators built at :330-334). */
+int bz_slab_left_mult(const double* P, int m, int k, const double* in, int nr, int nx, double* out,
+                      hipStream_t stream);
+
 /* Batched banded triangular solve: diags (bsz, ndiags, N), diagonal d of row i at
  * [d, i] (coupling row i to i-d if lower, i+d otherwise), ndiags <= 8.
  * Replaces lower/upper_banded_tri_batched and banded_tri_solve_batched
@@ -187,6 +203,45 @@ int bz_adv_x_be(const double* D_LtoR, const double* D_RtoL, const double* vr, co
  * as launched by SpSMSolver.solve (cusparse_spsolve.py:366-372). */
 int bz_bsr_upper_unit_solve(const double* data, const int* indptr, const int* indices, int nbr, int nrhs, int bsz,
                             const double* b, double* out, hipStream_t stream);
+
+/* ------------------------------------------------------------------ GMRES building blocks */
+
+/* out[j*ostride] (+)= sum_e V[j*ldv + e] u[e] for j < nv (or its square root), two-pass
+ * deterministic reduction over nblk partial sums (partial: nv*nblk doubles); out2[j] (if
+ * not NULL) receives the same sums.  Dot products and norms of utils.gmres_pcr
+ * (utils.py:1190, 1216-1222, 1230, 1236) batched over the Krylov basis. */
+int bz_multi_dot(const double* V, long ldv, int nv, const double* u, long n, double* partial, int nblk, double* out,
+                 int ostride, int accumulate, int take_sqrt, double* out2, hipStream_t stream);
+
+/* dst = src + sign * sum_{j<nv} coef[j*cstride] V[j*ldv + .]: the Gram-Schmidt update and
+ * x_n = x + V y of utils.gmres_pcr (utils.py:1218, 1229). */
+int bz_vt_axpy(const double* V, long ldv, int nv, const double* coef, int cstride, double sign, const double* src,
+               double* dst, long n, hipStream_t stream);
+
+/* y = x / s[0] (device scalar; nothing written when s[0] == 0): V[:, i+1] = u / H[i+1, i]
+ * (utils.py:1222-1223). */
+int bz_scale_inv(const double* x, const double* s, double* y, long n, hipStream_t stream);
+
+/* y[0..m-1] = argmin |H y - beta e_0| for the (m+1) x m Hessenberg H (row stride ldh),
+ * Givens rotations; work: (m+1)*m doubles, y: 2m+1 doubles.  Replaces the
+ * xp.linalg.lstsq call of utils.gmres_pcr (utils.py:1228). */
+int bz_hess_lstsq(const double* H, int ldh, int m, const double* beta, double* work, double* y, hipStream_t stream);
+
+/* ------------------------------------------------------------------ 1D3V v-space step */
+
+/* bin[x] = argmin_i |E[x] - Ev[i]| (first minimum): the preconditioner membership of
+ * glow1d_boltzmann.vspace_pc_setup (glowdischarge_boltzmann_1d.py:1429-1474). */
+int bz_nearest_bin(const double* E, int nx, const double* Ev, int nb, int* bin, hipStream_t stream);
+
+/* y[:, x] = P[bin[x]] u[:, x] for u, y (dof, nx), P (nbins, dof, dof): the per-point block
+ * preconditioner Mmat_mvec of step_bte_v (glowdischarge_boltzmann_1d.py:1526-1533). */
+int bz_binned_block_apply(const double* P, int dof, const int* bin, const double* u, int nx, double* y,
+                          hipStream_t stream);
+
+/* y = x - c (Kx + E[x] Ax) over (dof, nx) arrays: Lmat_mvec of step_bte_v
+ * (glowdischarge_boltzmann_1d.py:1520-1524) from the two operator products, c = dt tau. */
+int bz_vstep_combine(const double* x, const double* Kx, const double* Ax, const double* E, double c, int dof, int nx,
+                     double* y, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 def _args(E_by_N, Nr=40, lmax=1):
     return argparse.Namespace(collisions=XSEC_FILE, n0=3.22e22, Tg=300.0, E_field=E_by_N * 3.22e22 * 1e-21,
                               electron_volt=4.0, ev_max=80.0, Nr=Nr, l_max=lmax, sp_order=3, spline_qpts=5,
-                              ns_by_n0=[1.0], use_dg=0, vstype=0, ee_collisions=0, efield_period=0.0, ion_deg=0.0)
+                              ns_by_n0=[1.0, 1.0, 1.0], use_dg=0, vstype=0, ee_collisions=0, efield_period=0.0, ion_deg=0.0)
 
 
 def _oracle(args):
