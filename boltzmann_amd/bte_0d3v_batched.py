@@ -213,11 +213,42 @@ class bte_0d3v_batched:
         qoi = self.compute_QoIs(idx, h, effective_mobility=True)
         return h, qoi
 
+    def step(self, grid_idx: int, f0, atol, rtol, max_iter, time, delta_t):
+        """bte_0d3v_batched.py:981-1021: one backward-Euler step of size delta_t (seconds) from
+        f0 with the current parameters; raises RuntimeError where the reference exits."""
+        s = self._solver[grid_idx]
+        u = _dev_tensor(f0, self._dev).contiguous()
+        B = u.shape[1]
+        n0, Tg, E, ns = self._params(grid_idx, B)
+        x = u.clone()
+        sc = s.transient_step(x, n0, Tg, E, ns, float(delta_t), atol, rtol, max_iter, ustart=u)
+        if bool((sc["status"] != 0).any()):
+            raise RuntimeError(f"At time = {time:.2E} non-linear solver step FAILED: try a smaller time step "
+                               f"or more iterations")
+        return x
+
     def solve(self, grid_idx, f0, atol, rtol, max_iter, solver_type, alpha_min=1e-16, alpha_rho=0.5,
               line_search_c=1e-4, **kw):
         if solver_type == "steady-state":
             return self.steady_state_solve(grid_idx, f0, atol, rtol, max_iter, alpha_min, alpha_rho, line_search_c,
                                            **kw)
+        if solver_type == "transient":
+            # bte_0d3v_batched.py:1040-1161: returns (u, qoi of the last cycle average)
+            s = self._solver[grid_idx]
+            u0 = _dev_tensor(f0, self._dev).contiguous()
+            B = u0.shape[1]
+            n0, Tg, E, ns = self._params(grid_idx, B)
+            prm = self._par_bte_params[grid_idx]
+            if self._args.Efreq > 0:
+                eRe = _dev_tensor(prm["eRe"], self._dev).reshape(-1)
+                eIm = _dev_tensor(prm["eIm"], self._dev).reshape(-1)
+            else:
+                eRe, eIm = E, torch.zeros_like(E)
+            u, h, q, iters = s.transient(u0, n0, Tg, eRe, eIm, ns, self._args.dt, self._args.cycles,
+                                         self._args.Efreq, atol=atol, rtol=rtol, max_iter=max_iter)
+            self.last_info[grid_idx] = {"iters": iters}
+            return u, {"energy": q["energy"], "mobility": q["mobility"], "diffusion": q["diffusion"],
+                       "rates": q["rates"].contiguous()}
         raise NotImplementedError(solver_type)
 
     def normalized_distribution(self, grid_idx, ff):

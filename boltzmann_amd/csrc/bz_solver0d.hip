@@ -637,6 +637,150 @@ int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream) {
   return BZ_OK;
 }
 
+// One backward-Euler step of every instance (bte_0d3v_batched.solve "transient" and
+// newton_solver_batched, bte_0d3v_batched.py:59-126, 1068-1139):
+//   G(x) = (x - u) - c P (M^-1 K x - mu(x) x),  P = I - uhat uhat^T,  c = dt tau,
+// with x - u kept orthogonal to uhat.  The reference's Newton iteration freezes the
+// Jacobian I - c Q^T (M^-1 K - mu(u) I) Q at the start of the step and takes full steps.
+// In the original basis that step is
+//   (K - mu' M) z_G = M G,  (K - mu' M) z_u = M uhat,  mu' = mu(u) + 1/c,
+//   x <- x + (z_G - gamma z_u) / c,  gamma = uhat.z_G / uhat.z_u,
+// so every iteration factorises the same matrix K - mu' M with the two right-hand
+// sides, and the iterates are the reference's.  The iteration count and the stopping
+// test follow the reference loop (the convergence test of an iterate is made before
+// its update, and the update is still applied), per instance.
+template <int W, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_transient0d(GridDev g, BatchDev b,
+                                                                                           double cdt,
+                                                                                           const double* ustart,
+                                                                                           double* avg) {
+  extern __shared__ double lds[];
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  constexpr int WP = (W + 1) & ~1;
+  constexpr int PIVN = (W + 3) & ~1;
+  const int nchunk = (n + W - 1) / W;
+  double* X = lds;
+  double* Z1 = X + n;
+  double* Z2 = Z1 + n;
+  double* AV = Z2 + n + zpad(n, g.npr);
+  double* STG = AV + n;
+  double* PIV = STG + 2 * W * WP;
+  double* U0 = PIV + PIVN;
+  int* PO = reinterpret_cast<int*>(U0 + n);
+  short* FC = reinterpret_cast<short*>(PO + (n + 1));
+  short* NX = FC + n;
+  short* LEV = NX + n;
+  double* LCH = Z1;
+
+  double* kprof = b.kprof + (size_t)inst * g.P;
+  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+  const double* __restrict__ UH = g.uhat;
+  const double* __restrict__ WV = b.vec + (size_t)inst * 4 * n;
+  double* urows = b.urows + (size_t)inst * n * WP;
+
+  for (int i = lane; i < n; i += 64) {
+    FC[i] = (short)g.first_col[i];
+    NX[i] = (short)g.sched_next[i];
+    const size_t o = (size_t)orig_index(i, g.ns, g.npr) * b.B + inst;
+    X[i] = b.x[o];                              // initial iterate
+    U0[i] = ustart ? ustart[o] : b.x[o];        // state at the start of the step
+  }
+  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
+  for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
+  __syncthreads();
+
+  // frozen shift: the profile holds K; make it K - mu' M, mu' from the start state
+  const double mu0 = dot(WV, U0, n, lane);
+  const double mup = mu0 + 1.0 / cdt;
+  for (int k = lane; k < 7 * n; k += 64) {
+    const int i = k / 7, dd = k - 7 * (k / 7) - 3;
+    const int c = i + dd * g.ns;
+    if (c >= 0 && c < n) {
+      const int e = PO[i] + c - FC[i];
+      kprof[e] = fma(-mup, mprof[e], kprof[e]);
+    }
+  }
+  __syncthreads();
+
+  // G(X) into AV; returns |G|
+  auto residual = [&]() -> double {
+    for (int e = lane; e < 4 * g.npr; e += 64) {
+      const int q = e >> 2, dd = e & 3;
+      const double v = g.mchol[(size_t)(q * g.ns) * 4 + dd];
+      LCH[e] = (dd == 0) ? 1.0 / v : ((q - dd >= 0) ? v : 0.0);
+    }
+    for (int i = lane; i < n; i += 64) AV[i] = 0.0;
+    __syncthreads();
+    flat_matvec(g.P, kprof, g.ent_rc, X, AV, lane);
+    __syncthreads();
+    mass_solve(g.ns, g.npr, LCH, AV, lane);
+    __syncthreads();
+    const double mu = dot(WV, X, n, lane);
+    double su = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double a = fma(mup, X[i], AV[i]) - mu * X[i];      // M^-1 K x - mu x
+      AV[i] = a;
+      su = fma(UH[i], a, su);
+    }
+    su = wave_sum(su);
+    double s2 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double gi = (X[i] - U0[i]) - cdt * (AV[i] - su * UH[i]);
+      AV[i] = gi;
+      s2 = fma(gi, gi, s2);
+    }
+    __syncthreads();
+    return sqrt(wave_sum(s2));
+  };
+
+  const double r0 = residual();
+  double rr = r0;
+  bool conv = (r0 < b.atol);
+  int it = 0;
+  while (!conv && it < b.max_iter) {
+    if (it > 0) rr = residual();
+    conv = (rr / r0 < b.rtol) || (rr < b.atol);
+    for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
+    mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, AV, Z2, lane);
+    __syncthreads();
+    factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
+    __syncthreads();
+    back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
+    __syncthreads();
+    const double gam = dot(UH, Z2, n, lane) / dot(UH, Z1, n, lane);
+    const double ic = 1.0 / cdt;
+    for (int i = lane; i < n; i += 64) X[i] = fma(ic, Z2[i] - gam * Z1[i], X[i]);
+    __syncthreads();
+    ++it;
+  }
+
+  for (int i = lane; i < n; i += 64) {
+    const size_t o = (size_t)orig_index(i, g.ns, g.npr) * b.B + inst;
+    b.x[o] = X[i];
+    if (avg) avg[o] += U0[i] + X[i];
+  }
+  if (lane == 0) {
+    b.iters[inst] = it;
+    b.status[inst] = conv ? 0 : 1;
+    b.res_abs[inst] = rr;
+    b.res0[inst] = r0;
+  }
+}
+
+template <int W, int R>
+int launch_transient(const GridDev& g, const BatchDev& b, double cdt, const double* ustart, double* avg,
+                     hipStream_t stream) {
+  const size_t lds = newton_lds_bytes<W>(g.n, g.npr) + (size_t)g.n * sizeof(double);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  const long work = (long)b.B * g.P;
+  const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_prepare0d, dim3(blocks), dim3(256), 0, stream, g, b);
+  BZ_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_transient0d<W, R>), dim3(b.B), dim3(64), lds, stream, g, b, cdt, ustart, avg);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
 // Gather dense (n x n, original order) operators into permuted row-profile storage.
 __global__ void k_gather_profile(const double* __restrict__ dense, int n, int ns, int npr,
                                  const int* __restrict__ first_col, const int* __restrict__ prof_off,
@@ -726,6 +870,27 @@ __global__ void k_mchol_perm(const double* __restrict__ lrev, int ns, int npr, d
   for (int d = 0; d < 4; ++d) out[i * 4 + d] = lrev[q * 4 + d];
 }
 
+// Copy the C-ABI structures into the kernel argument structures and check what every
+// solver kernel relies on.
+int to_dev(const bz_grid0d_t* gp, const bz_batch0d_t* bp, GridDev& g, BatchDev& b) {
+  g.n = gp->n; g.ns = gp->ns; g.npr = gp->npr; g.bw = gp->bw; g.S = gp->S; g.nop = gp->nop; g.P = gp->P;
+  g.nslots = gp->nslots;
+  g.first_col = gp->first_col; g.prof_off = gp->prof_off; g.opprof = gp->opprof; g.slot_first = gp->slot_first;
+  g.sched_next = gp->sched_next; g.muhat = gp->muhat; g.uhat = gp->uhat; g.u = gp->u; g.wsp = gp->wsp;
+  g.mchol = gp->mchol; g.chunk_levels = gp->chunk_levels; g.ent_rc = gp->ent_rc;
+  if (g.n != g.ns * g.npr || g.n < 8 || g.n > 32767 || g.nop != g.S + 3 || g.nop > kMaxOps || g.S < 1)
+    return BZ_EINVAL;
+  if (g.nslots < 64 || g.nslots % 64 != 0) return BZ_EINVAL;
+  b.B = bp->B; b.n0 = bp->n0; b.Tg = bp->Tg; b.E = bp->E; b.ns_by_n0 = bp->ns_by_n0; b.x = bp->x;
+  b.kprof = bp->kprof; b.urows = bp->urows; b.vec = bp->vec; b.iters = bp->iters; b.status = bp->status;
+  b.res_abs = bp->res_abs; b.res0 = bp->res0; b.atol = bp->atol; b.rtol = bp->rtol; b.alpha_min = bp->alpha_min;
+  b.alpha_rho = bp->alpha_rho; b.ls_c = bp->ls_c; b.max_iter = bp->max_iter; b.mode = bp->mode;
+  b.tg_scale = bp->tg_scale;
+  if (b.B < 1 || b.max_iter < 0) return BZ_EINVAL;
+  if (!(b.atol >= 0.0) || !(b.rtol >= 0.0)) return BZ_EINVAL;
+  return BZ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -762,29 +927,39 @@ int bz_mchol_perm(const double* lrev, int ns, int npr, double* out, hipStream_t 
   return BZ_OK;
 }
 
+int bz_transient_step_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, double dt_tau, const double* ustart,
+                         double* avg, hipStream_t stream) {
+  if (!gp || !bp || !(dt_tau > 0.0)) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  const int Wneed = g.bw + 1;
+  const int R = g.nslots / 64;
+  if (Wneed <= 8) {
+    if (R == 1) return launch_transient<8, 1>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 2) return launch_transient<8, 2>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 3) return launch_transient<8, 3>(g, b, dt_tau, ustart, avg, stream);
+  } else if (Wneed <= 11) {
+    if (R == 1) return launch_transient<11, 1>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 2) return launch_transient<11, 2>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 3) return launch_transient<11, 3>(g, b, dt_tau, ustart, avg, stream);
+  } else if (Wneed <= 17) {
+    if (R == 2) return launch_transient<17, 2>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 3) return launch_transient<17, 3>(g, b, dt_tau, ustart, avg, stream);
+    if (R == 4) return launch_transient<17, 4>(g, b, dt_tau, ustart, avg, stream);
+  }
+  return BZ_EINVAL;
+}
+
 int bz_steady_solve_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, hipStream_t stream) {
   if (!gp || !bp) return BZ_EINVAL;
   GridDev g;
-  g.n = gp->n; g.ns = gp->ns; g.npr = gp->npr; g.bw = gp->bw; g.S = gp->S; g.nop = gp->nop; g.P = gp->P;
-  g.nslots = gp->nslots;
-  g.first_col = gp->first_col; g.prof_off = gp->prof_off; g.opprof = gp->opprof; g.slot_first = gp->slot_first;
-  g.sched_next = gp->sched_next; g.muhat = gp->muhat; g.uhat = gp->uhat; g.u = gp->u; g.wsp = gp->wsp;
-  g.mchol = gp->mchol; g.chunk_levels = gp->chunk_levels; g.ent_rc = gp->ent_rc;
-  if (g.n != g.ns * g.npr || g.n < 8 || g.n > 32767 || g.nop != g.S + 3 || g.nop > kMaxOps || g.S < 1)
-    return BZ_EINVAL;
   BatchDev b;
-  b.B = bp->B; b.n0 = bp->n0; b.Tg = bp->Tg; b.E = bp->E; b.ns_by_n0 = bp->ns_by_n0; b.x = bp->x;
-  b.kprof = bp->kprof; b.urows = bp->urows; b.vec = bp->vec; b.iters = bp->iters; b.status = bp->status;
-  b.res_abs = bp->res_abs; b.res0 = bp->res0; b.atol = bp->atol; b.rtol = bp->rtol; b.alpha_min = bp->alpha_min;
-  b.alpha_rho = bp->alpha_rho; b.ls_c = bp->ls_c; b.max_iter = bp->max_iter; b.mode = bp->mode;
-  b.tg_scale = bp->tg_scale;
-  if (b.B < 1 || b.max_iter < 0) return BZ_EINVAL;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
   // the backtracking loop terminates only for 0 < alpha_rho < 1 and alpha_min > 0
   if (!(b.alpha_rho > 0.0 && b.alpha_rho < 1.0) || !(b.alpha_min > 0.0)) return BZ_EINVAL;
-  if (!(b.atol >= 0.0) || !(b.rtol >= 0.0)) return BZ_EINVAL;
   const int Wneed = g.bw + 1;
-  const int R = (g.nslots + 63) / 64;
-  if (g.nslots != 64 * R) return BZ_EINVAL;
+  const int R = g.nslots / 64;
   if (Wneed <= 8) {
     if (R == 1) return launch_newton<8, 1>(g, b, stream);
     if (R == 2) return launch_newton<8, 2>(g, b, stream);

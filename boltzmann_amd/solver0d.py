@@ -92,7 +92,67 @@ class Batched0D:
         _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
         return sc
 
-    def qois(self, h, n0, E, effective_mobility=True, stream=None):
+    def transient_step(self, x, n0, Tg, E, ns_by_n0, dt_tau, atol=1e-10, rtol=1e-10, max_iter=30, ustart=None,
+                       avg=None, tg_scale=1.0, stream=None):
+        """One backward-Euler step of size dt_tau for every instance (bz_transient_step_0d):
+        x (n, B) holds the initial iterate on entry and the new state on exit; ustart is the
+        state at the start of the step (None: x itself); avg += ustart + x_new if given.
+        Returns the scratch dict (iters, status, res_abs, res0)."""
+        B = x.shape[1]
+        assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
+        for v in (n0, Tg, E):
+            assert v.shape == (B,) and v.dtype == torch.float64 and v.is_contiguous()
+        assert ns_by_n0.shape == (self.S, B) and ns_by_n0.is_contiguous()
+        for v in (ustart, avg):
+            assert v is None or (v.shape == x.shape and v.is_contiguous() and v.dtype == torch.float64)
+        sc = self._get_scratch(B)
+        bt = _lib.Batch0D(B=B, n0=n0.data_ptr(), Tg=Tg.data_ptr(), E=E.data_ptr(), ns_by_n0=ns_by_n0.data_ptr(),
+                          x=x.data_ptr(), kprof=sc["kprof"].data_ptr(), urows=sc["urows"].data_ptr(),
+                          vec=sc["vec"].data_ptr(), iters=sc["iters"].data_ptr(), status=sc["status"].data_ptr(),
+                          res_abs=sc["res_abs"].data_ptr(), res0=sc["res0"].data_ptr(), atol=float(atol),
+                          rtol=float(rtol), alpha_min=1e-16, alpha_rho=0.5, ls_c=0.0, max_iter=int(max_iter), mode=0,
+                          tg_scale=float(tg_scale))
+        _lib.call("bz_transient_step_0d", self.tables.struct, bt, float(dt_tau), ustart, avg, stream=stream)
+        return sc
+
+    def transient(self, f0, n0, Tg, eRe, eIm, ns_by_n0, dt, cycles, Efreq, atol=1e-10, rtol=1e-10, max_iter=30):
+        """bte_0d3v_batched.solve(..., "transient") (bte_0d3v_batched.py:1040-1161): backward
+        Euler in RF periods (step dt * tau, tau = 1/Efreq), E(t) = eRe cos 2 pi t + eIm sin 2 pi t,
+        each step's Newton started from the previous step's increment, trapezoidal average over
+        each cycle; returns (u, h_avg, qoi of the last full cycle's average, newton iterations)."""
+        tau = 1e-7 if Efreq == 0 else 1.0 / Efreq
+        io_cycle = 1.0
+        io_freq = int(io_cycle / dt)
+        steps_total = int(cycles / dt)
+        u = f0 / (self.u @ f0)[None, :]
+        u = u.contiguous()
+        u_prev = u.clone()
+        avg = torch.zeros_like(u)
+        avg_v = torch.zeros_like(u)
+        E = torch.empty_like(n0)
+        x = torch.empty_like(u)
+        tt = 0.0
+        iters = []
+        for ts_idx in range(steps_total + 1):
+            if ts_idx > 0 and ts_idx % io_freq == 0:
+                avg.mul_(0.5 * dt / io_cycle)
+                avg_v.copy_(avg)
+                avg.zero_()
+                if ts_idx == steps_total:
+                    break
+            ph = 2.0 * math.pi * (tt + dt)
+            if Efreq > 0:
+                torch.add(eRe * math.cos(ph), eIm, alpha=math.sin(ph), out=E)
+            else:
+                E.copy_(eRe)
+            # initial iterate u + (u - u_prev): the previous step's increment
+            torch.sub(2.0 * u, u_prev, out=x)
+            sc = self.transient_step(x, n0, Tg, E, ns_by_n0, dt * tau, atol, rtol, max_iter, ustart=u, avg=avg)
+            iters.append(sc["iters"].clone())
+            u_prev, u = u, x.clone()
+            tt += dt
+        h, q = self.qois(avg_v.contiguous(), n0, torch.ones_like(n0), effective_mobility=False)
+        return u, h, q, torch.stack(iters) if iters else None
         """Normalised distribution and QoIs (bte_0d3v_batched.py:1163-1217)."""
         g = self.g
         n, B = h.shape

@@ -125,6 +125,18 @@ typedef struct {
  * (bte_0d3v_batched.py:602-821, 856-979). */
 int bz_steady_solve_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, hipStream_t stream);
 
+/* One backward-Euler step of size dt_tau (= dt * tau) for all B instances from the state
+ * ustart (n, B), normalised u.x = 1 (NULL: batch->x), with batch->x the initial iterate on
+ * entry and the new state on exit: the Newton iteration of newton_solver_batched with the
+ * Jacobian frozen at the start of the step and full steps, batch->E the field at the end
+ * of the step, batch->atol / rtol / max_iter its stopping rule (per instance); iters,
+ * status (0 converged, 1 iteration cap), res_abs (last |G|) and res0 are written.  If avg
+ * is not NULL, avg (n, B) += x_old + x_new (the reference's cycle average accumulator).
+ * Replaces bte_0d3v_batched.solve(..., "transient") inner step and newton_solver_batched
+ * (bte_0d3v_batched.py:59-126, 1068-1139) and bte_0d3v_batched.step (:981-1021). */
+int bz_transient_step_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, double dt_tau, const double* ustart,
+                         double* avg, hipStream_t stream);
+
 /* Window width (columns) the solver uses for an upper bandwidth bw (-1: unsupported). */
 int bz_solver0d_window(int bw);
 

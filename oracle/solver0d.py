@@ -8,6 +8,9 @@ Restates (BESolver/python/...):
   * single path _QRQt, newton_solve,
     steady_state_solver, compute_QoIs        bte_0d3v_solver.py:404-460, 24-73, 741-859, 1234-1276
 
+  * newton_solver_batched, step, solve("transient")
+                                             bte_0d3v_batched.py:59-126, 981-1161
+
 Pinned by tests/test_oracle_solver.py: the single-path fixed point equals the
 generalized eigenvector of (C + E A, M) with u^T h = 1 (scipy.linalg.eig);
 the batched fixed point satisfies the same eigen relation up to the
@@ -323,3 +326,75 @@ def single_qois(ops: SingleOps, hh, E_field, effective_mobility=True):
     D = (hh[:, 0::ns] @ ops.diffusion) * (c / 3.0)
     rr = [hh[:, 0::ns] @ ops.rates[i] for i in range(len(ops.procs))]
     return {"energy": mu, "mobility": M, "diffusion": D, "rates": rr}
+
+
+# ----------------------------------------------------------------------------- transient
+
+def newton_solver_batched(x, residual, jac_inv, atol, rtol, iter_max):
+    """bte_0d3v_batched.py:59-126: frozen Jacobian inverse, full steps; the convergence
+    test of an iterate is made before its update and the update is still applied."""
+    r0 = residual(x)
+    norm_r0 = np.linalg.norm(r0, axis=0)
+    norm_rr = norm_r0
+    count = 0
+    converged = (norm_rr / norm_r0 < rtol).all() or (norm_rr < atol).all()
+    while not converged and count < iter_max:
+        rr = residual(x)
+        norm_rr = np.linalg.norm(rr, axis=0)
+        converged = (norm_rr / norm_r0 < rtol).all() or (norm_rr < atol).all()
+        p = np.einsum("ijk,ki->ji", jac_inv, -rr)
+        x = x + p
+        count += 1
+    return {"status": converged, "x": x, "atol": norm_rr, "rtol": norm_rr / norm_r0, "iter": count}
+
+
+def transient_step(ops: BatchedOps, params, u, dt_tau, atol, rtol, max_iter, du0=None):
+    """One backward-Euler step (bte_0d3v_batched.py:981-1021 with dt -> dt * tau as in
+    solve("transient"), :1083-1093): u (n, B) normalised, du0 the initial increment in the
+    Q basis (zero in step(); the previous step's increment in solve("transient"));
+    returns (u_new, info)."""
+    res, jac = batched_res_jac(ops, params, "reference")
+    Q = ops.Q
+    B = u.shape[1]
+    I = np.eye(Q.shape[1])
+    J = I[None] - dt_tau * jac(u)
+    Jinv = np.linalg.inv(J)
+    du = np.zeros((Q.shape[1], B)) if du0 is None else du0
+    info = newton_solver_batched(du, lambda d: d - dt_tau * res(u + Q @ d), Jinv, atol, rtol, max_iter)
+    return u + Q @ info["x"], info
+
+
+def transient_solve(ops: BatchedOps, params, f0, dt, cycles, Efreq, atol, rtol, max_iter):
+    """bte_0d3v_batched.solve(..., "transient") (:1040-1161): E(t) = eRe cos 2 pi t +
+    eIm sin 2 pi t (t in RF periods), dt in periods, step dt * tau with tau = 1/Efreq,
+    trapezoidal cycle average of the last full cycle; returns (u, qoi of the average)."""
+    tau = 1e-7 if Efreq == 0 else 1.0 / Efreq
+    io_cycle = 1.0
+    io_freq = int(io_cycle / dt)
+    steps_total = int(cycles / dt)
+    u = f0 / (ops.u @ f0)
+    tt = 0.0
+    cycle_avg_u = np.zeros_like(u)
+    cycle_avg_v = np.zeros_like(u)
+    du = np.zeros((ops.Q.shape[1], u.shape[1]))
+    for ts_idx in range(steps_total + 1):
+        prm = dict(params)
+        if Efreq > 0:
+            prm["E"] = params["eRe"] * np.cos(2 * np.pi * (tt + dt)) + params["eIm"] * np.sin(2 * np.pi * (tt + dt))
+        if ts_idx > 0 and ts_idx % io_freq == 0:
+            cycle_avg_u *= 0.5 * dt / io_cycle
+            cycle_avg_v[:] = cycle_avg_u
+            cycle_avg_u[:] = 0
+            if ts_idx == steps_total:
+                break
+        u_new, info = transient_step(ops, prm, u, dt * tau, atol, rtol, max_iter, du0=du)
+        if not info["status"]:
+            raise RuntimeError("transient Newton step failed")
+        du = info["x"]
+        cycle_avg_u += u
+        u = u_new
+        cycle_avg_u += u
+        tt += dt
+    h = normalized_distribution(ops, cycle_avg_v)
+    q = compute_qois(ops, params, h, effective_mobility=False)
+    return u, q
