@@ -153,6 +153,8 @@ class Batched0D:
             tt += dt
         h, q = self.qois(avg_v.contiguous(), n0, torch.ones_like(n0), effective_mobility=False)
         return u, h, q, torch.stack(iters) if iters else None
+
+    def qois(self, h, n0, E, effective_mobility=True, stream=None):
         """Normalised distribution and QoIs (bte_0d3v_batched.py:1163-1217)."""
         g = self.g
         n, B = h.shape
