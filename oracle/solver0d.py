@@ -98,17 +98,33 @@ def batched_res_jac(ops: BatchedOps, params, jacobian="reference"):
     n0, Tg, E, ns_by_n0 = params["n0"], params["Tg"], params["E"], params["ns_by_n0"]
     I = np.eye(QT.shape[0])
 
+    ee = ops.ee
+    if ee is not None:
+        from . import ee as eeo
+        cc = ee["cc"]
+        cc12 = np.swapaxes(cc, 1, 2)
+        ne = params["ne"]
+
     def res(x):
-        return (n0 * (np.einsum("sx,svx->vx", ns_by_n0, np.einsum("svu,ux->svx", QT_Cen, x))
-                      + Tg * (QT_Cgt @ x))
-                + E * (QT_A @ x)
-                - n0 * np.einsum("sx,sx->x", ns_by_n0, W @ x) * (QT @ x))
+        y = (n0 * (np.einsum("sx,svx->vx", ns_by_n0, np.einsum("svu,ux->svx", QT_Cen, x))
+                   + Tg * (QT_Cgt @ x))
+             + E * (QT_A @ x)
+             - n0 * np.einsum("sx,sx->x", ns_by_n0, W @ x) * (QT @ x))
+        if ee is not None:
+            # + ne gamma_a(x) Q^T (cc x) x   (bte_0d3v_batched.py:702-709)
+            y = y + ne * eeo.gamma_a(ops, x, ne) * (QT @ eeo.ee_term(cc, x))
+        return y
 
     def jac(x):
         ns = n0 * ns_by_n0
         L = np.einsum("sx,svu->xvu", ns, QT_Cen_Q) + np.einsum("a,bc->abc", n0 * Tg, QT_Cgt_Q)
         mu = np.einsum("sx,sx->x", ns, W @ x)
         J = L + np.einsum("a,bc->abc", E, QT_A_Q) - np.einsum("a,bc->abc", mu, I)
+        if ee is not None:
+            # + ne gamma_a Q^T (cc x + cc_12 x) Q   (bte_0d3v_batched.py:734-743)
+            c1 = np.tensordot(cc, x, axes=([2], [0])) + np.tensordot(cc12, x, axes=([2], [0]))   # (n, n, B)
+            QTcQ = np.einsum("bi,ijx,jc->xbc", QT, c1, Q)
+            J = J + np.einsum("x,xbc->xbc", ne * eeo.gamma_a(ops, x, ne), QTcQ)
         if jacobian == "exact":
             wq = np.einsum("sx,su,uk->xk", ns, W, Q)
             J = J - np.einsum("vx,xk->xvk", QT @ x, wq)

@@ -1,0 +1,128 @@
+"""Pins of the oracle's e-e Fokker-Planck operator (oracle/ee.py) and of the weak-form
+term table (boltzmann_amd/ee_terms_table.py):
+
+* the committed table equals a fresh symbolic derivation (lmax = 1), and its (0,0,0)
+  entries are the textbook ones: Ia = v^2 B_p' B_r' B_k / (2 sqrt(pi)),
+  Ib = B_p' B_r' B_k / (2 sqrt(pi)) + v^2 B_p'' B_r'' B_k / (4 sqrt(pi));
+* partial moments Pm / Qm against scipy.integrate.quad;
+* the potentials of a Maxwellian are the analytic Rosenbluth potentials
+  h ~ erf(v)/v and g ~ (v + 1/(2v)) erf(v) + exp(-v^2)/sqrt(pi) (one constant ratio);
+* C(f, f) conserves particles exactly, momentum and energy to the discretisation error
+  (decreasing with resolution), and vanishes on a Maxwellian to second order.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.integrate
+
+from helpers import oracle_procs
+from oracle import bspline, ee
+from oracle import operators as op
+from oracle import solver0d as s0
+
+
+def test_table_matches_derivation_and_textbook_terms():
+    from boltzmann_amd.ee_terms_gen import derive
+    from boltzmann_amd.ee_terms_table import TERMS
+    fresh = derive(1)
+    for kind in ("a", "b"):
+        for idx, terms in fresh[kind].items():
+            got = sorted(TERMS[kind][idx])
+            ref = sorted(terms)
+            assert len(got) == len(ref)
+            for a, b in zip(got, ref):
+                assert a[1:] == b[1:] and a[0] == pytest.approx(b[0], rel=1e-14)
+    c = 1.0 / (2.0 * math.sqrt(math.pi))
+
+    def as_dict(terms):
+        return {t[1:]: t[0] for t in terms}
+    ta, tb = as_dict(TERMS["a"][(0, 0, 0)]), as_dict(TERMS["b"][(0, 0, 0)])
+    assert set(ta) == {(2, 1, 0, 1)} and ta[(2, 1, 0, 1)] == pytest.approx(c, rel=1e-15)
+    assert set(tb) == {(0, 1, 0, 1), (2, 2, 0, 2)}
+    assert tb[(0, 1, 0, 1)] == pytest.approx(c, rel=1e-15) and tb[(2, 2, 0, 2)] == pytest.approx(c / 2, rel=1e-15)
+
+
+@pytest.fixture(scope="module")
+def grids():
+    procs = oracle_procs()
+    out = {}
+    for Nr in (16, 32):
+        g = op.grid_batched(Nr, 1, 4.0, 80.0, 5, [p.threshold for p in procs])
+        ops = s0.BatchedOps(g, procs)
+        from boltzmann_amd.ee_terms_table import TERMS
+        hl, gl = ee.rosenbluth_ops(g, ops.Minv)
+        A, Bt = ee.cc_tensors(g, TERMS)
+        out[Nr] = (g, ops, hl, gl, A, Bt)
+    return out
+
+
+def test_partial_moments_against_quad(grids):
+    g = grids[16][0]
+    gx, _ = g.quad(g.q)
+    P = ee.Pm(g, 3, gx)
+    Q = ee.Qm(g, 1, gx)
+    for i, j in ((0, 3), (5, 30), (10, 60), (16, len(gx) - 1)):
+        f = lambda v: bspline.basis_eval(g.t, 3, i, np.array([v]))[0]
+        lo, hi = g.t[i], g.t[i + 4]
+        x = gx[j]
+        kw = dict(limit=400, epsabs=0.0, epsrel=1e-13)
+        a, b = lo, min(max(x, lo), hi)
+        pref = scipy.integrate.quad(lambda v: v ** 3 * f(v), a, b, points=[k for k in g.t if a < k < b], **kw)[0] \
+            if x > lo else 0.0
+        a, b = max(x, lo), hi
+        qref = scipy.integrate.quad(lambda v: f(v) / v, a, b, points=[k for k in g.t if a < k < b], **kw)[0] \
+            if x < hi else 0.0
+        assert P[i, j] == pytest.approx(pref, rel=1e-10, abs=1e-14)
+        assert Q[i, j] == pytest.approx(qref, rel=1e-10, abs=1e-14)
+
+
+def test_maxwellian_potentials_are_analytic(grids):
+    g, ops, hl, gl, A, Bt = grids[32]
+    ns = g.num_sh
+    f = ops.initial(1)[:, 0]
+    gx, _ = g.quad(g.q)
+    V = g.V(gx)
+    h = V.T @ (hl @ f)[0::ns]
+    gg = V.T @ (gl @ f)[0::ns]
+    ha = np.array([2 * math.erf(x) / x for x in gx])
+    ga = np.array([(x + 1 / (2 * x)) * math.erf(x) + math.exp(-x * x) / math.sqrt(math.pi) for x in gx])
+    for num, ana in ((h, ha), (gg, ga)):
+        r = num / ana
+        # L2 projection of a smooth potential: agreement to the spline approximation error
+        assert np.ptp(r) <= 5e-5 * abs(r.mean())
+        assert r.mean() == pytest.approx(1.0 / math.sqrt(math.pi), rel=2e-5)
+    # no l = 1 potential for an isotropic distribution
+    assert np.abs((hl @ f)[1::ns]).max() == 0.0
+
+
+def _collision(A, Bt, hl, gl, f):
+    Ta = np.einsum("ijk,j,k->i", A, f, hl @ f)
+    Tb = np.einsum("ijk,j,k->i", Bt, f, gl @ f)
+    return Ta, Ta + Tb
+
+
+def test_conservation_and_maxwellian_null_space(grids):
+    errs = {}
+    for Nr, (g, ops, hl, gl, A, Bt) in grids.items():
+        ns = g.num_sh
+        t = g.t
+        xi = np.array([(t[p + 1] + t[p + 2] + t[p + 3]) / 3 for p in range(g.num_p)])   # v = sum xi_p B_p
+        xs = np.linspace(t[0], t[-1], 400)
+        c2 = np.linalg.lstsq(g.V(xs).T, xs ** 2, rcond=None)[0]                           # v^2
+        f = ops.initial(1)[:, 0]
+        rng = np.random.default_rng(1)
+        fa = f.copy()
+        fa[1::ns] = 0.2 * f[0::ns] * rng.uniform(0.5, 1.5, g.num_p)
+        fa[0::ns] *= 1 + 0.2 * rng.normal(size=g.num_p)
+        Ta, T = _collision(A, Bt, hl, gl, fa)
+        scale = np.abs(Ta).sum()
+        assert abs(T[0::ns].sum()) <= 1e-13 * scale                      # particles, exactly
+        errs[Nr] = (abs((xi * T[1::ns]).sum()) / scale, abs((c2 * T[0::ns]).sum()) / scale)
+        Tm_a, Tm = _collision(A, Bt, hl, gl, f)
+        errs[Nr] += (np.linalg.norm(Tm) / np.linalg.norm(Tm_a),)
+    # momentum and energy to the discretisation error, falling with resolution
+    assert errs[16][0] < 1e-4 and errs[32][0] < errs[16][0] / 8
+    assert errs[16][1] < 1e-4 and errs[32][1] < errs[16][1] / 8
+    # Maxwellian null space: second order in the knot spacing
+    assert errs[32][2] < 0.01 and errs[32][2] < errs[16][2] / 3.5
