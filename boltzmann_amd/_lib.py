@@ -35,7 +35,12 @@ class Batch0D(ctypes.Structure):
                 ("x", c_void_p), ("kprof", c_void_p), ("urows", c_void_p), ("vec", c_void_p), ("iters", c_void_p),
                 ("status", c_void_p), ("res_abs", c_void_p), ("res0", c_void_p), ("atol", c_double),
                 ("rtol", c_double), ("alpha_min", c_double), ("alpha_rho", c_double), ("ls_c", c_double),
-                ("max_iter", c_int), ("mode", c_int), ("tg_scale", c_double)]
+                ("max_iter", c_int), ("mode", c_int), ("tg_scale", c_double), ("tol_abs", c_void_p)]
+
+
+class EE0D(ctypes.Structure):
+    _fields_ = [("ns", c_int), ("npr", c_int), ("ccA", c_void_p), ("ccB", c_void_p), ("h", c_void_p), ("g", c_void_p),
+                ("negam", c_void_p)]
 
 
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;

@@ -1,0 +1,284 @@
+// Electron-electron Fokker-Planck operator in the B-spline x harmonic basis (gfx950, FP64).
+//
+// Assembly (once per grid):
+//   * partial moments of every B-spline at the quadrature points (Rosenbluth potentials,
+//     collision_operator_spherical.py:786-909), the potential kernels m1, m2 per l and the
+//     potential operators H_l, G_l = M^-1 [int v^2 B_p m(., k)] (:911-970);
+//   * the trilinear weak-form tensors cc_a, cc_b (:973-1050) stored banded:
+//     cc[p][k-p+3][r-p+3][q][l][s] for |k-p|, |r-p| <= 3 (three cubic B-splines overlap only
+//     there), 49 ns^3 values per radial index instead of the dense n^3 tensor.
+// Per Newton step of the batched solver (the factored form of bte_0d3v_batched.py:444-476,
+// 702-709): potentials h = H x, g = G x of every instance and the Coulomb prefactor
+// ne gamma_a(x) (:652-661); the e-e operator at fixed potentials is then added to the
+// instance's K profile by k_ee_profile_add (bz_solver0d.hip).
+#include <algorithm>
+
+#include "bz_common.h"
+
+namespace {
+
+using namespace bz;
+
+// Values, first and second derivatives of the 4 cubic B-splines nonzero on interval l
+// (t[l] <= x < t[l+1]): N[r], D1[r], D2[r] for B_{l-3+r}.
+__device__ void bspline3_d2(const double* __restrict__ t, int l, double x, double N[4], double D1[4], double D2[4]) {
+  // order 1 (linear) on interval l: B_{l-1,1}, B_{l,1}
+  const double h0 = t[l + 1] - t[l];
+  const double b1[2] = {(t[l + 1] - x) / h0, (x - t[l]) / h0};
+  // derivatives of order-2 B_{j,2}, j = l-2, l-1, l: B'_{j,2} = 2 (B_{j,1}/(t[j+2]-t[j]) - B_{j+1,1}/(t[j+3]-t[j+1]))
+  double d2o[3];
+  for (int a = 0; a < 3; ++a) {
+    const int j = l - 2 + a;
+    const double bj = (a >= 1) ? b1[a - 1] : 0.0;      // B_{j,1}
+    const double bj1 = (a <= 1) ? b1[a] : 0.0;         // B_{j+1,1}
+    const double e1 = t[j + 2] - t[j], e2 = t[j + 3] - t[j + 1];
+    double v = 0.0;
+    if (e1 != 0.0) v += 2.0 * bj / e1;
+    if (e2 != 0.0) v -= 2.0 * bj1 / e2;
+    d2o[a] = v;
+  }
+  bspline3_eval(t, l, x, N, D1);
+  for (int r = 0; r < 4; ++r) {
+    const int i = l - 3 + r;
+    const double di = (r >= 1) ? d2o[r - 1] : 0.0;      // B'_{i,2}
+    const double di1 = (r <= 2) ? d2o[r] : 0.0;         // B'_{i+1,2}
+    const double e1 = t[i + 3] - t[i], e2 = t[i + 4] - t[i + 1];
+    double v = 0.0;
+    if (e1 != 0.0) v += 3.0 * di / e1;
+    if (e2 != 0.0) v -= 3.0 * di1 / e2;
+    D2[r] = v;
+  }
+}
+
+__device__ __forceinline__ double ipow(double x, int m) {
+  double r = 1.0;
+  const bool neg = m < 0;
+  int e = neg ? -m : m;
+  double b = x;
+  while (e) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return neg ? 1.0 / r : r;
+}
+
+// sum over the knot pieces of supp(B_i) clipped to [xb, xe] of int v^m B_i, qn Gauss points
+// per piece (integrate_xm_splines, collision_operator_spherical.py:803-817)
+__device__ double piece_integral(const double* __restrict__ t, int npr, int i, int m, double xb, double xe,
+                                 const double* __restrict__ qx, const double* __restrict__ qw, int qn) {
+  double s = 0.0;
+  for (int kn = i + 1; kn <= i + 4; ++kn) {
+    const double a = fmax(xb, t[kn - 1]), b = fmin(xe, t[kn]);
+    if (b > a) {
+      const double hw = 0.5 * (b - a), c = 0.5 * (b + a);
+      const int l = kn - 1;                       // knot interval [t[l], t[l+1]]
+      for (int g = 0; g < qn; ++g) {
+        const double x = hw * qx[g] + c;
+        double N[4], D[4];
+        bspline3_eval(t, l, x, N, D);
+        const int r = i - (l - 3);
+        s += hw * qw[g] * ipow(x, m) * N[r];
+      }
+    }
+  }
+  return s;
+}
+
+// kind 0: P[i, j] = int_{t_i}^{min(x_j, t_(i+4))} v^m B_i  (0 below the support)
+// kind 1: Q[i, j] = int_{max(x_j, t_i)}^{t_(i+4)} v^-m B_i (0 above the support)
+__global__ void k_partial_moment(const double* __restrict__ t, int npr, const double* __restrict__ gx, int nq, int m,
+                                 int kind, const double* __restrict__ qx, const double* __restrict__ qw, int qn,
+                                 double* __restrict__ out) {
+  const long total = (long)npr * nq;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / nq), j = (int)(e % nq);
+    const double lo = t[i], hi = t[i + 4], x = gx[j];
+    double v = 0.0;
+    if (kind == 0) {
+      if (x >= lo) v = piece_integral(t, npr, i, m, lo, fmin(x, hi), qx, qw, qn);
+    } else {
+      if (x <= hi) v = piece_integral(t, npr, i, -m, fmax(x, lo), hi, qx, qw, qn);
+    }
+    out[e] = v;
+  }
+}
+
+// m1, m2 of harmonic l at the points (collision_operator_spherical.py:915-920)
+__global__ void k_potential_kernels(int l, double m_ab, const double* __restrict__ gx, int nq, int npr,
+                                    const double* __restrict__ P2, const double* __restrict__ P4,
+                                    const double* __restrict__ Q1, const double* __restrict__ Q3, double* __restrict__ m1,
+                                    double* __restrict__ m2) {
+  const long total = (long)npr * nq;
+  const double f1 = (l - 0.5) / (l + 1.5);
+  const double c1 = (2.0 / (2 * l + 1)) * (1.0 + m_ab);
+  const double c2 = -2.0 / (4.0 * l * l - 1.0);
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const double x = gx[e % nq];
+    m1[e] = c1 * (P2[e] / ipow(x, l + 1) + Q1[e] * ipow(x, l));
+    m2[e] = c2 * ((P2[e] / ipow(x, l - 1) - f1 * P4[e] / ipow(x, l + 1)) + (Q3[e] * ipow(x, l) - f1 * Q1[e] * ipow(x, l + 2)));
+  }
+}
+
+// C[i, j] = alpha * sum_k A[i*sai + k*sak] B[k*sbk + j*sbj]  (small set-up products)
+__global__ void k_gemm_strided(int m, int n, int kk, double alpha, const double* __restrict__ A, long sai, long sak,
+                               const double* __restrict__ B, long sbk, long sbj, double* __restrict__ C, long ldc) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double s = 0.0;
+    for (int k = 0; k < kk; ++k) s = fma(A[i * sai + k * sak], B[k * sbk + j * sbj], s);
+    C[(long)i * ldc + j] = alpha * s;
+  }
+}
+
+// Banded trilinear tensors.  One thread per (p, dk, dr): the points of every knot interval
+// where B_p, B_k and B_r overlap (q Gauss points per interval), all (q, l, s) combinations.
+// Term tables: combination c = (q*ns + l)*ns + s has monomials toff[c] .. toff[c+1]-1 of
+// (coef, power of v, derivative orders of B_p, B_k, B_r); kind a and b are separate tables.
+__global__ void k_cc_banded(const double* __restrict__ t, int npr, int ns, const double* __restrict__ glx,
+                            const double* __restrict__ glw, int q, const int* __restrict__ ta_off,
+                            const double* __restrict__ ta_c, const int* __restrict__ ta_i,
+                            const int* __restrict__ tb_off, const double* __restrict__ tb_c,
+                            const int* __restrict__ tb_i, double* __restrict__ ccA, double* __restrict__ ccB) {
+  const int ncomb = ns * ns * ns;
+  const long total = (long)npr * 49;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(e / 49), dk = (int)((e / 7) % 7) - 3, dr = (int)(e % 7) - 3;
+    const int k = p + dk, r = p + dr;
+    double* oa = ccA + e * ncomb;
+    double* ob = ccB + e * ncomb;
+    for (int c = 0; c < ncomb; ++c) { oa[c] = 0.0; ob[c] = 0.0; }
+    if (k < 0 || k >= npr || r < 0 || r >= npr) continue;
+    // intervals l with all three supports: l in [max(p,k,r), min(p,k,r)+3]
+    const int l0 = max(p, max(k, r)), l1 = min(p, min(k, r)) + 3;
+    for (int l = l0; l <= l1; ++l) {
+      if (!(t[l + 1] > t[l])) continue;
+      const double hw = 0.5 * (t[l + 1] - t[l]), cx = 0.5 * (t[l + 1] + t[l]);
+      for (int g = 0; g < q; ++g) {
+        const double x = hw * glx[g] + cx, w = hw * glw[g];
+        double N[4], D1[4], D2[4];
+        bspline3_d2(t, l, x, N, D1, D2);
+        const int rp = p - (l - 3), rk = k - (l - 3), rr = r - (l - 3);
+        const double bp[3] = {N[rp], D1[rp], D2[rp]};
+        const double bk[3] = {N[rk], D1[rk], D2[rk]};
+        const double br[3] = {N[rr], D1[rr], D2[rr]};
+        for (int c = 0; c < ncomb; ++c) {
+          double sa = 0.0;
+          for (int m = ta_off[c]; m < ta_off[c + 1]; ++m)
+            sa += ta_c[m] * ipow(x, ta_i[4 * m]) * bp[ta_i[4 * m + 1]] * bk[ta_i[4 * m + 2]] * br[ta_i[4 * m + 3]];
+          double sb = 0.0;
+          for (int m = tb_off[c]; m < tb_off[c + 1]; ++m)
+            sb += tb_c[m] * ipow(x, tb_i[4 * m]) * bp[tb_i[4 * m + 1]] * bk[tb_i[4 * m + 2]] * br[tb_i[4 * m + 3]];
+          oa[c] += w * sa;
+          ob[c] += w * sb;
+        }
+      }
+    }
+  }
+}
+
+// h[b, r*ns + s] = sum_k H[s][r][k] x[(k*ns + s), b]  (and g with G); x (n, B), h, g (B, n)
+__global__ void k_ee_potentials(const double* __restrict__ H, const double* __restrict__ G, int ns, int npr,
+                                const double* __restrict__ x, int B, double* __restrict__ h, double* __restrict__ g) {
+  const int n = ns * npr;
+  const long total = (long)n * B;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int b = (int)(e % B), row = (int)(e / B);
+    const int r = row / ns, s = row % ns;
+    const double* Hr = H + ((long)s * npr + r) * npr;
+    const double* Gr = G + ((long)s * npr + r) * npr;
+    double sh = 0.0, sg = 0.0;
+    for (int k = 0; k < npr; ++k) {
+      const double xv = x[(long)(k * ns + s) * B + b];
+      sh = fma(Hr[k], xv, sh);
+      sg = fma(Gr[k], xv, sg);
+    }
+    h[(long)b * n + row] = sh;
+    g[(long)b * n + row] = sg;
+  }
+}
+
+// negam[b] = ne[b] * gamma_a(x_b): m0 = c_m0 (mass . x), kT = |c_kt (top . x) / m0|,
+// Lambda = c_lam kT^1.5 / sqrt(ne), gamma = ln(Lambda) c_gam  (bte_0d3v_batched.py:652-661)
+__global__ void k_ee_gamma(const double* __restrict__ x, int n, int B, const double* __restrict__ mass,
+                           const double* __restrict__ top, const double* __restrict__ ne, double c_m0, double c_kt,
+                           double c_lam, double c_gam, double* __restrict__ negam) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double sm = 0.0, st = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double xv = x[(long)i * B + b];
+    sm = fma(mass[i], xv, sm);
+    st = fma(top[i], xv, st);
+  }
+  const double m0 = c_m0 * sm;
+  const double kT = fabs(c_kt * st / m0);
+  const double lam = c_lam * kT * sqrt(kT) / sqrt(ne[b]);
+  negam[b] = ne[b] * log(lam) * c_gam;
+}
+
+int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 16384); }
+
+}  // namespace
+
+extern "C" {
+
+int bz_partial_moment(const double* t, int npr, const double* gx, int nq, int m, int kind, const double* qx,
+                      const double* qw, int qn, double* out, hipStream_t stream) {
+  if (npr < 4 || nq < 1 || qn < 1 || (kind != 0 && kind != 1) || !t || !gx || !qx || !qw || !out) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_partial_moment, dim3(grid_for((long)npr * nq)), dim3(256), 0, stream, t, npr, gx, nq, m, kind, qx,
+                     qw, qn, out);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_potential_kernels(int l, double m_ab, const double* gx, int nq, int npr, const double* P2, const double* P4,
+                         const double* Q1, const double* Q3, double* m1, double* m2, hipStream_t stream) {
+  if (l < 0 || nq < 1 || npr < 1 || !gx || !P2 || !P4 || !Q1 || !Q3 || !m1 || !m2) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_potential_kernels, dim3(grid_for((long)npr * nq)), dim3(256), 0, stream, l, m_ab, gx, nq, npr, P2,
+                     P4, Q1, Q3, m1, m2);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_gemm_strided(int m, int n, int k, double alpha, const double* A, long sai, long sak, const double* B, long sbk,
+                    long sbj, double* C, long ldc, hipStream_t stream) {
+  if (m < 1 || n < 1 || k < 1 || !A || !B || !C || ldc < n) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_gemm_strided, dim3(grid_for((long)m * n)), dim3(256), 0, stream, m, n, k, alpha, A, sai, sak, B,
+                     sbk, sbj, C, ldc);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_cc_banded(const double* t, int npr, int ns, const double* glx, const double* glw, int q, const int* ta_off,
+                 const double* ta_c, const int* ta_i, const int* tb_off, const double* tb_c, const int* tb_i,
+                 double* ccA, double* ccB, hipStream_t stream) {
+  if (npr < 4 || ns < 1 || ns > 8 || q < 1 || !t || !glx || !glw || !ta_off || !ta_c || !ta_i || !tb_off || !tb_c ||
+      !tb_i || !ccA || !ccB)
+    return BZ_EINVAL;
+  hipLaunchKernelGGL(k_cc_banded, dim3(grid_for((long)npr * 49)), dim3(256), 0, stream, t, npr, ns, glx, glw, q, ta_off,
+                     ta_c, ta_i, tb_off, tb_c, tb_i, ccA, ccB);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h, double* g,
+                     hipStream_t stream) {
+  if (ns < 1 || npr < 1 || B < 1 || !H || !G || !x || !h || !g) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_potentials, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B, h,
+                     g);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double* top, const double* ne, double c_m0,
+                double c_kt, double c_lam, double c_gam, double* negam, hipStream_t stream) {
+  if (n < 1 || B < 1 || !x || !mass || !top || !ne || !negam) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_gamma, dim3((B + 255) / 256), dim3(256), 0, stream, x, n, B, mass, top, ne, c_m0, c_kt, c_lam,
+                     c_gam, negam);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+}  // extern "C"

@@ -66,6 +66,7 @@ struct BatchDev {
   double atol, rtol, alpha_min, alpha_rho, ls_c;
   int max_iter, mode;       // mode 0: reference Jacobian (d = 0), 1: exact Jacobian
   double tg_scale;          // coefficient of C_Tg is n0 * Tg * tg_scale
+  const double* tol_abs;    // [B] per-instance absolute tolerance, or null: atol + rtol * r0
 };
 
 __device__ __forceinline__ int orig_index(int i, int ns, int npr) {
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const double r0 = proj_res_plain(AV, X, UH, n, mu, lane);
   double rcur = r0;
   double abs_err = r0;
-  const double tol = b.atol + b.rtol * r0;
+  const double tol = b.tol_abs ? b.tol_abs[inst] : b.atol + b.rtol * r0;
   int it = 0, status = 1, verify = 0;
 
   for (;;) {
@@ -624,14 +625,57 @@ size_t newton_lds_bytes(int n, int npr) {
          (size_t)(2 * n + nchunk) * sizeof(short);
 }
 
+struct EEDev {
+  const double* ccA;    // [npr][7][7][ns][ns][ns]
+  const double* ccB;
+  const double* h;      // (B, n) original order
+  const double* g;
+  const double* negam;  // [B]
+};
+
+// K profile += ne gamma_a [cc_a(., h) + cc_b(., g)]: the e-e operator of every instance at
+// its current potentials, entry (row (p,q), column (k,l)) = sum_{r,s} cc[(p,q),(k,l),(r,s)] h[(r,s)]
+// (the M-space form of cc = M^-1 (cc_a hl_op + cc_b gl_op), bte_0d3v_batched.py:462-468).
+__global__ void k_ee_profile_add(GridDev g, BatchDev b, EEDev e) {
+  const long total = (long)b.B * g.P;
+  const int ns = g.ns, n = g.n, npr = g.npr;
+  const int nc = ns * ns * ns;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int inst = (int)(idx / g.P);
+    const int ent = (int)(idx - (long)inst * g.P);
+    const unsigned rc = g.ent_rc[ent];
+    const int oi = orig_index((int)(rc >> 16), ns, npr), oj = orig_index((int)(rc & 0xffffu), ns, npr);
+    const int p = oi / ns, q = oi - p * ns, k = oj / ns, l = oj - k * ns;
+    const int dk = k - p;
+    if (dk < -3 || dk > 3) continue;
+    const double* hv = e.h + (size_t)inst * n;
+    const double* gv = e.g + (size_t)inst * n;
+    double s = 0.0;
+    for (int dr = -3; dr <= 3; ++dr) {
+      const int r = p + dr;
+      if (r < 0 || r >= npr) continue;
+      const size_t base = (((size_t)p * 7 + (dk + 3)) * 7 + (dr + 3)) * nc + (size_t)(q * ns + l) * ns;
+      for (int ss = 0; ss < ns; ++ss) {
+        s = fma(e.ccA[base + ss], hv[r * ns + ss], s);
+        s = fma(e.ccB[base + ss], gv[r * ns + ss], s);
+      }
+    }
+    b.kprof[idx] = fma(e.negam[inst], s, b.kprof[idx]);
+  }
+}
+
 template <int W, int R>
-int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream) {
+int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream, const EEDev* ee = nullptr) {
   const size_t lds = newton_lds_bytes<W>(g.n, g.npr);
   if (lds > 64 * 1024) return BZ_EINVAL;
   const long work = (long)b.B * g.P;
   const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
   hipLaunchKernelGGL(k_prepare0d, dim3(blocks), dim3(256), 0, stream, g, b);
   BZ_LAUNCH_CHECK();
+  if (ee) {
+    hipLaunchKernelGGL(k_ee_profile_add, dim3(blocks), dim3(256), 0, stream, g, b, *ee);
+    BZ_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL((k_newton0d<W, R>), dim3(b.B), dim3(64), lds, stream, g, b);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
@@ -886,6 +930,7 @@ int to_dev(const bz_grid0d_t* gp, const bz_batch0d_t* bp, GridDev& g, BatchDev& 
   b.res_abs = bp->res_abs; b.res0 = bp->res0; b.atol = bp->atol; b.rtol = bp->rtol; b.alpha_min = bp->alpha_min;
   b.alpha_rho = bp->alpha_rho; b.ls_c = bp->ls_c; b.max_iter = bp->max_iter; b.mode = bp->mode;
   b.tg_scale = bp->tg_scale;
+  b.tol_abs = bp->tol_abs;
   if (b.B < 1 || b.max_iter < 0) return BZ_EINVAL;
   if (!(b.atol >= 0.0) || !(b.rtol >= 0.0)) return BZ_EINVAL;
   return BZ_OK;
@@ -951,29 +996,44 @@ int bz_transient_step_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, double d
   return BZ_EINVAL;
 }
 
-int bz_steady_solve_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, hipStream_t stream) {
-  if (!gp || !bp) return BZ_EINVAL;
-  GridDev g;
-  BatchDev b;
-  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+static int steady_dispatch(const GridDev& g, const BatchDev& b, hipStream_t stream, const EEDev* ee) {
   // the backtracking loop terminates only for 0 < alpha_rho < 1 and alpha_min > 0
   if (!(b.alpha_rho > 0.0 && b.alpha_rho < 1.0) || !(b.alpha_min > 0.0)) return BZ_EINVAL;
   const int Wneed = g.bw + 1;
   const int R = g.nslots / 64;
   if (Wneed <= 8) {
-    if (R == 1) return launch_newton<8, 1>(g, b, stream);
-    if (R == 2) return launch_newton<8, 2>(g, b, stream);
-    if (R == 3) return launch_newton<8, 3>(g, b, stream);
+    if (R == 1) return launch_newton<8, 1>(g, b, stream, ee);
+    if (R == 2) return launch_newton<8, 2>(g, b, stream, ee);
+    if (R == 3) return launch_newton<8, 3>(g, b, stream, ee);
   } else if (Wneed <= 11) {
-    if (R == 1) return launch_newton<11, 1>(g, b, stream);
-    if (R == 2) return launch_newton<11, 2>(g, b, stream);
-    if (R == 3) return launch_newton<11, 3>(g, b, stream);
+    if (R == 1) return launch_newton<11, 1>(g, b, stream, ee);
+    if (R == 2) return launch_newton<11, 2>(g, b, stream, ee);
+    if (R == 3) return launch_newton<11, 3>(g, b, stream, ee);
   } else if (Wneed <= 17) {
-    if (R == 2) return launch_newton<17, 2>(g, b, stream);
-    if (R == 3) return launch_newton<17, 3>(g, b, stream);
-    if (R == 4) return launch_newton<17, 4>(g, b, stream);
+    if (R == 1) return launch_newton<17, 1>(g, b, stream, ee);
+    if (R == 2) return launch_newton<17, 2>(g, b, stream, ee);
+    if (R == 3) return launch_newton<17, 3>(g, b, stream, ee);
+    if (R == 4) return launch_newton<17, 4>(g, b, stream, ee);
   }
   return BZ_EINVAL;
+}
+
+int bz_steady_solve_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, hipStream_t stream) {
+  if (!gp || !bp) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  return steady_dispatch(g, b, stream, nullptr);
+}
+
+int bz_steady_solve_0d_ee(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const bz_ee0d_t* ep, hipStream_t stream) {
+  if (!gp || !bp || !ep) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  if (ep->ns != g.ns || ep->npr != g.npr || !ep->ccA || !ep->ccB || !ep->h || !ep->g || !ep->negam) return BZ_EINVAL;
+  EEDev e{ep->ccA, ep->ccB, ep->h, ep->g, ep->negam};
+  return steady_dispatch(g, b, stream, &e);
 }
 
 int bz_solver0d_window(int bw) {

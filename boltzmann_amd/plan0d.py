@@ -117,10 +117,11 @@ def plan_structure(mask_orig, npr, ns, window_fn):
 class SolverTables:
     """Device tables of one grid for bz_steady_solve_0d."""
 
-    def __init__(self, vgrid, ops, growth_mask, u_orig, mass_radial):
+    def __init__(self, vgrid, ops, growth_mask, u_orig, mass_radial, extra_mask=None):
         """vgrid: VGrid; ops: list of dense (n,n) device tensors [C_0..C_{S-1}, C_Tg, A_E, M];
         growth_mask[m] (m < nop-1): operator m contributes to the growth rate mu = w.x;
-        u_orig: mass vector u (original order, device); mass_radial: radial mass block."""
+        u_orig: mass vector u (original order, device); mass_radial: radial mass block;
+        extra_mask: (n, n) bool pattern of per-instance operators added later (e-e)."""
         dev = vgrid.dev
         self.dev = dev
         npr, ns, n = vgrid.npr, vgrid.num_sh, vgrid.n
@@ -130,6 +131,8 @@ class SolverTables:
         mask = torch.zeros((n, n), dtype=torch.bool, device=dev)
         for A in ops:
             mask |= (A != 0)
+        if extra_mask is not None:
+            mask |= extra_mask.to(dev)
         mask_np = mask.cpu().numpy()
         L = _lib.lib()
         st = plan_structure(mask_np, npr, ns, lambda bw: L.bz_solver0d_window(bw))

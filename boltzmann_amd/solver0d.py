@@ -18,7 +18,7 @@ from .vspace import C_GAMMA, QE, ME, TEMP_EV_FAC, VGrid, XSection
 
 
 class Batched0D:
-    def __init__(self, vgrid: VGrid, xsecs, growth="collisions"):
+    def __init__(self, vgrid: VGrid, xsecs, growth="collisions", ee_pattern=False):
         """growth: 'collisions' (batched path: W_s = u M^-1 C_s, bte_0d3v_batched.py:636) or
         'all' (single path: W = u M^-1 (C + E A), bte_0d3v_solver.py:787)."""
         self.g = vgrid
@@ -52,7 +52,13 @@ class Batched0D:
         self.mw0 = 1.0 / (math.sqrt(math.pi) * g.vth) ** 3
         self.u = (self.mass * (self.mw0 * g.vth ** 3)).contiguous()
         gm = [True] * self.S + ([False, False] if growth == "collisions" else [True, True])
-        self.tables = SolverTables(g, C + [Cg, A, M], gm, self.u, self.mass_radial)
+        self.ee_pattern = ee_pattern
+        extra = None
+        if ee_pattern:
+            # e-e couples (p, q) to (k, l) for |k - p| <= 3 and every pair of harmonics
+            pr = torch.arange(n, device=dev) // g.num_sh
+            extra = (pr[:, None] - pr[None, :]).abs() <= 3
+        self.tables = SolverTables(g, C + [Cg, A, M], gm, self.u, self.mass_radial, extra_mask=extra)
         self.h0 = g.maxwellian_init()
         self._scratch = {}
 
@@ -74,22 +80,32 @@ class Batched0D:
         return self._scratch[B]
 
     def solve_raw(self, x, n0, Tg, E, ns_by_n0, atol=1e-10, rtol=1e-10, max_iter=30, alpha_min=1e-16,
-                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None):
-        """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0)."""
+                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None, tol_abs=None, ee=None):
+        """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0).
+        tol_abs: optional (B,) per-instance absolute tolerance; ee: optional _lib.EE0D with the
+        frozen e-e operator of every instance (bz_steady_solve_0d_ee)."""
         B = x.shape[1]
         assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
         for v in (n0, Tg, E):
             assert v.shape == (B,) and v.dtype == torch.float64 and v.is_contiguous()
         assert ns_by_n0.shape == (self.S, B) and ns_by_n0.is_contiguous()
+        if tol_abs is not None:
+            assert tol_abs.shape == (B,) and tol_abs.dtype == torch.float64 and tol_abs.is_contiguous()
         sc = self._get_scratch(B)
         bt = _lib.Batch0D(B=B, n0=n0.data_ptr(), Tg=Tg.data_ptr(), E=E.data_ptr(), ns_by_n0=ns_by_n0.data_ptr(),
                           x=x.data_ptr(), kprof=sc["kprof"].data_ptr(), urows=sc["urows"].data_ptr(),
                           vec=sc["vec"].data_ptr(), iters=sc["iters"].data_ptr(), status=sc["status"].data_ptr(),
                           res_abs=sc["res_abs"].data_ptr(), res0=sc["res0"].data_ptr(), atol=float(atol),
                           rtol=float(rtol), alpha_min=float(alpha_min), alpha_rho=float(alpha_rho),
-                          ls_c=float(ls_c), max_iter=int(max_iter), mode=int(mode), tg_scale=float(tg_scale))
+                          ls_c=float(ls_c), max_iter=int(max_iter), mode=int(mode), tg_scale=float(tg_scale),
+                          tol_abs=None if tol_abs is None else tol_abs.data_ptr())
         self._last_batch = bt
-        _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
+        if ee is None:
+            _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
+        else:
+            if not self.ee_pattern:
+                raise _lib.BzError("the solver tables were planned without the e-e pattern (ee_pattern=True)")
+            _lib.call("bz_steady_solve_0d_ee", self.tables.struct, bt, ee, stream=stream)
         return sc
 
     def transient_step(self, x, n0, Tg, E, ns_by_n0, dt_tau, atol=1e-10, rtol=1e-10, max_iter=30, ustart=None,

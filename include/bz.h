@@ -117,6 +117,7 @@ typedef struct {
   int max_iter;
   int mode;               /* 0: reference Jacobian, 1: exact Jacobian          */
   double tg_scale;        /* C_Tg coefficient is n0 * Tg * tg_scale            */
+  const double* tol_abs;  /* [B] per-instance absolute tolerance; NULL: atol + rtol * |r0| */
 } bz_batch0d_t;
 
 /* Batched damped-Newton steady state of all B instances in one launch (one wavefront
@@ -136,6 +137,25 @@ int bz_steady_solve_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, hipSt
  * (bte_0d3v_batched.py:59-126, 1068-1139) and bte_0d3v_batched.step (:981-1021). */
 int bz_transient_step_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, double dt_tau, const double* ustart,
                          double* avg, hipStream_t stream);
+
+/* e-e operator data of one batch (device pointers): the banded trilinear tensors of the
+ * grid and the current potentials and Coulomb prefactor of every instance. */
+typedef struct {
+  int ns, npr;
+  const double* ccA;      /* [npr][7][7][ns][ns][ns]: cc_a[(p,q),(p+dk-3,l),(p+dr-3,s)]  */
+  const double* ccB;      /* same layout for cc_b                                     */
+  const double* h;        /* (B, n) first Rosenbluth potential of every instance       */
+  const double* g;        /* (B, n) second potential                                   */
+  const double* negam;    /* [B]   ne * gamma_a                                         */
+} bz_ee0d_t;
+
+/* bz_steady_solve_0d with the e-e operator at the given (frozen) potentials added to every
+ * instance's operator: K + ne gamma_a [cc_a(., h) + cc_b(., g)] (M-space form of
+ * cc = M^-1 (cc_a hl_op + cc_b gl_op)).  Outer iterations over the potentials converge to
+ * the fixed point of the reference's e-e residual (bte_0d3v_batched.py:702-709, 882-979).
+ * The grid tables must be planned with the e-e pattern (|k - p| <= 3 for all l, q). */
+int bz_steady_solve_0d_ee(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const bz_ee0d_t* ee,
+                          hipStream_t stream);
 
 /* Window width (columns) the solver uses for an upper bandwidth bw (-1: unsupported). */
 int bz_solver0d_window(int bw);
@@ -254,6 +274,44 @@ int bz_binned_block_apply(const double* P, int dof, const int* bin, const double
  * (glowdischarge_boltzmann_1d.py:1520-1524) from the two operator products, c = dt tau. */
 int bz_vstep_combine(const double* x, const double* Kx, const double* Ax, const double* E, double c, int dof, int nx,
                      double* y, hipStream_t stream);
+
+/* ------------------------------------------------------------------ e-e assembly */
+
+/* Partial moments at the points gx: kind 0: out[i*nq+j] = int_{t_i}^{min(x_j, t_(i+4))} v^m B_i
+ * (0 for x_j below the support); kind 1: int_{max(x_j, t_i)}^{t_(i+4)} v^-m B_i (0 above it);
+ * qn Gauss points (qx, qw) per knot piece.  Pm / Qm of
+ * CollisionOpSP.compute_rosenbluth_potentials_op (collision_operator_spherical.py:819-909). */
+int bz_partial_moment(const double* t, int npr, const double* gx, int nq, int m, int kind, const double* qx,
+                      const double* qw, int qn, double* out, hipStream_t stream);
+
+/* Potential kernels m1, m2 (npr x nq) of harmonic l from P(l+2), P(l+4), Q(l-1), Q(l-3)
+ * (collision_operator_spherical.py:911-920). */
+int bz_potential_kernels(int l, double m_ab, const double* gx, int nq, int npr, const double* P2, const double* P4,
+                         const double* Q1, const double* Q3, double* m1, double* m2, hipStream_t stream);
+
+/* C[i*ldc + j] = alpha sum_k A[i*sai + k*sak] B[k*sbk + j*sbj] (set-up products of the
+ * potential operators, collision_operator_spherical.py:940-970). */
+int bz_gemm_strided(int m, int n, int k, double alpha, const double* A, long sai, long sak, const double* B, long sbk,
+                    long sbj, double* C, long ldc, hipStream_t stream);
+
+/* Banded trilinear tensors cc_a, cc_b ([npr][7][7][ns][ns][ns]) from the weak-form term tables
+ * (combination c = (q*ns+l)*ns+s: monomials off[c]..off[c+1]-1 with coefficients c[] and
+ * 4 ints (power of v, derivative orders of B_p, B_k, B_r)), q Gauss points per interval.
+ * Replaces CollisionOpSP.coulomb_collision_op_assembly (collision_operator_spherical.py:973-1050). */
+int bz_cc_banded(const double* t, int npr, int ns, const double* glx, const double* glw, int q, const int* ta_off,
+                 const double* ta_c, const int* ta_i, const int* tb_off, const double* tb_c, const int* tb_i,
+                 double* ccA, double* ccB, hipStream_t stream);
+
+/* Potentials of every instance: h[b, r*ns+s] = sum_k H[s][r][k] x[(k*ns+s), b] (and g with G);
+ * H, G: (ns, npr, npr) blocks of hl_op, gl_op; x (n, B); h, g (B, n).  The products
+ * hl_op x, gl_op x inside cc x x (bte_0d3v_batched.py:462-468, 702-703). */
+int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h, double* g,
+                     hipStream_t stream);
+
+/* negam[b] = ne[b] gamma_a(x_b): m0 = c_m0 mass.x, kT = |c_kt top.x / m0|,
+ * Lambda = c_lam kT^1.5 / sqrt(ne), gamma_a = ln(Lambda) c_gam (bte_0d3v_batched.py:652-661). */
+int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double* top, const double* ne, double c_m0,
+                double c_kt, double c_lam, double c_gam, double* negam, hipStream_t stream);
 
 #ifdef __cplusplus
 }
