@@ -117,6 +117,7 @@ class EEOperator:
         self.c_lam = 12.0 * math.pi * eps0 ** 1.5 / qe ** 3
         self.c_gam = qe ** 4 / (4.0 * math.pi * (eps0 * me) ** 2) / vth ** 3
         self.top = (solver.temp * TEMP_K_1EV).contiguous()
+        self.anderson_depth = 8
 
     def potentials(self, x, ne):
         """h, g (B, n) and ne * gamma_a (B,) of the distributions x (n, B)."""
@@ -130,7 +131,7 @@ class EEOperator:
         return h, g, negam
 
     def steady_state(self, x, n0, Tg, E, ns, ne, atol, rtol, max_iter, alpha_min=1e-16, alpha_rho=0.5,
-                     line_search_c=1e-4, inner_iter=3, max_outer=200, mode=1):
+                     line_search_c=1e-4, inner_iter=1, max_outer=400, mode=1):
         """In place on x (n, B).  Returns a dict with per-instance iters (inner Newton steps),
         status (0 converged), res_abs (full residual at the end), res0 (full residual at the
         start) and the number of outer iterations."""
@@ -144,22 +145,39 @@ class EEOperator:
         max_outer = min(max_outer, max(int(max_iter), 1) + 1)
         # the potentials are those of the normalised iterate (u.x = 1), as in the reference
         x.div_((s.u @ x)[None, :])
+        m_hist = self.anderson_depth
+        n = self.n
+        dX = torch.zeros((m_hist, n, B), dtype=torch.float64, device=self.dev)
+        dF = torch.zeros_like(dX)
+        x_prev = f_prev = None
+        nh = 0
         while outer < max_outer:
             h, g, negam = self.potentials(x, ne)
             ee = _lib.EE0D(ns=self.ns, npr=self.npr, ccA=self.ccA.data_ptr(), ccB=self.ccB.data_ptr(),
                            h=h.data_ptr(), g=g.data_ptr(), negam=negam.data_ptr())
+            x_old = x.clone()
             sc_ = s.solve_raw(x, n0, Tg, E, ns, atol=atol, rtol=rtol, max_iter=0 if tol is None else inner_iter,
                               alpha_min=alpha_min, alpha_rho=alpha_rho, ls_c=line_search_c, mode=mode, tol_abs=tol,
                               ee=ee)
             res = sc_["res0"].clone()               # full residual at the entry iterate
+            outer += 1
             if tol is None:
                 r0 = res
                 tol = (atol + rtol * r0).contiguous()
-                outer += 1
                 continue
             iters += sc_["iters"]
-            outer += 1
             if bool((res <= tol).all()):
+                x.copy_(x_old)
                 break
+            # Anderson mixing of the fixed-point map x -> (frozen-potential Newton steps)(x)
+            f = x - x_old
+            if x_prev is not None:
+                slot = nh % m_hist
+                torch.sub(x_old, x_prev, out=dX[slot])
+                torch.sub(f, f_prev, out=dF[slot])
+                nh += 1
+            x_prev, f_prev = x_old, f
+            x.copy_(x_old)
+            _lib.call("bz_anderson_mix", n, B, min(nh, m_hist), dX, dF, f.contiguous(), x)
         status = (res > tol).to(torch.int32)
         return {"iters": iters, "status": status, "res_abs": res, "res0": r0, "outer": outer}

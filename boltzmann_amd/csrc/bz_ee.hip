@@ -220,6 +220,63 @@ __global__ void k_ee_gamma(const double* __restrict__ x, int n, int B, const dou
 
 int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 16384); }
 
+// Anderson mixing of the outer potential iteration, per instance b (columns of (n, B)
+// arrays).  History slots j = 0..m-1 hold dX_j = x_(j+1) - x_j and dF_j = f_(j+1) - f_j
+// (ring buffer, slot stride n*B); f = Phi(x) - x of the current iterate.
+// gamma = argmin |f - dF gamma| by the normal equations (m <= 16, Cholesky with a relative
+// ridge of 1e-13 on the diagonal), x <- x + f - (dX + dF) gamma.
+__global__ void k_anderson(int n, int B, int m, const double* __restrict__ dX, const double* __restrict__ dF,
+                           const double* __restrict__ f, double* __restrict__ x) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double G[16 * 16], r[16];
+  const long slot = (long)n * B;
+  for (int i = 0; i < m; ++i) {
+    double ri = 0.0;
+    for (int e = 0; e < n; ++e) ri = fma(dF[i * slot + (long)e * B + b], f[(long)e * B + b], ri);
+    r[i] = ri;
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+      for (int e = 0; e < n; ++e) s = fma(dF[i * slot + (long)e * B + b], dF[j * slot + (long)e * B + b], s);
+      G[i * 16 + j] = s;
+      G[j * 16 + i] = s;
+    }
+  }
+  double tr = 0.0;
+  for (int i = 0; i < m; ++i) tr += G[i * 16 + i];
+  const double ridge = 1e-13 * tr + 1e-300;
+  for (int i = 0; i < m; ++i) G[i * 16 + i] += ridge;
+  // Cholesky G = L L^T in place (lower part), then solve
+  for (int j = 0; j < m; ++j) {
+    double d = G[j * 16 + j];
+    for (int k = 0; k < j; ++k) d -= G[j * 16 + k] * G[j * 16 + k];
+    d = sqrt(fmax(d, 1e-300));
+    G[j * 16 + j] = d;
+    for (int i = j + 1; i < m; ++i) {
+      double s = G[i * 16 + j];
+      for (int k = 0; k < j; ++k) s -= G[i * 16 + k] * G[j * 16 + k];
+      G[i * 16 + j] = s / d;
+    }
+  }
+  double gm[16];
+  for (int i = 0; i < m; ++i) {
+    double s = r[i];
+    for (int k = 0; k < i; ++k) s -= G[i * 16 + k] * gm[k];
+    gm[i] = s / G[i * 16 + i];
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double s = gm[i];
+    for (int k = i + 1; k < m; ++k) s -= G[k * 16 + i] * gm[k];
+    gm[i] = s / G[i * 16 + i];
+  }
+  for (int e = 0; e < n; ++e) {
+    const long o = (long)e * B + b;
+    double v = x[o] + f[o];
+    for (int j = 0; j < m; ++j) v -= gm[j] * (dX[j * slot + o] + dF[j * slot + o]);
+    x[o] = v;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,6 +325,14 @@ int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const do
   if (ns < 1 || npr < 1 || B < 1 || !H || !G || !x || !h || !g) return BZ_EINVAL;
   hipLaunchKernelGGL(k_ee_potentials, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B, h,
                      g);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_anderson_mix(int n, int B, int m, const double* dX, const double* dF, const double* f, double* x,
+                    hipStream_t stream) {
+  if (n < 1 || B < 1 || m < 0 || m > 16 || !f || !x || (m > 0 && (!dX || !dF))) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_anderson, dim3((B + 63) / 64), dim3(64), 0, stream, n, B, m, dX, dF, f, x);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

@@ -313,6 +313,14 @@ int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const do
 int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double* top, const double* ne, double c_m0,
                 double c_kt, double c_lam, double c_gam, double* negam, hipStream_t stream);
 
+/* Anderson mixing step of the outer e-e potential iteration for every instance (columns of
+ * (n, B) arrays): gamma = argmin |f - dF gamma| over the m <= 16 history differences
+ * (dX, dF: m slots of (n, B)), x <- x + f - (dX + dF) gamma.  No reference counterpart: the
+ * reference differentiates the potentials inside its dense Jacobian
+ * (bte_0d3v_batched.py:734-743); this accelerates the banded outer iteration that replaces it. */
+int bz_anderson_mix(int n, int B, int m, const double* dX, const double* dF, const double* f, double* x,
+                    hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,7 +90,7 @@ def test_steady_state_with_ee(setup):
         ops.ee = None
     x = _d(x0)
     n0, Tg, E, ns, ne = (_d(prm[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
-    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-12, 200)
+    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-12, 300)
     torch.cuda.synchronize()
     print("outer iterations", out["outer"], "inner", out["iters"].cpu().numpy(), flush=True)
     assert (out["status"].cpu().numpy() == 0).all()
