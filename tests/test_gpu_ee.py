@@ -85,18 +85,23 @@ def test_steady_state_with_ee(setup):
     x0, q0, _ = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, 1e-13, 40, jacobian="exact")
     ops.ee = {"cc": eeo.ee_operator(ops.Minv, A, Bt, hl, gl)}
     try:
-        x_ref, q_ref, info = s0.steady_state_solve_each(ops, prm, x0, 0.0, 1e-12, 80)
+        # the oracle's Newton with the reference's e-e Jacobian (cc x + cc_12 x) plus the
+        # growth derivative, to the FP64 floor: the fixed point both solvers approach
+        x_ref, q_ref, info = s0.steady_state_solve_each(ops, prm, x0, 0.0, 1e-14, 60, jacobian="exact")
     finally:
         ops.ee = None
     x = _d(x0)
     n0, Tg, E, ns, ne = (_d(prm[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
-    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-12, 300)
+    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-12, 400)
     torch.cuda.synchronize()
     print("outer iterations", out["outer"], "inner", out["iters"].cpu().numpy(), flush=True)
     assert (out["status"].cpu().numpy() == 0).all()
     h, q = s.qois(x, n0, E)
     qg = {k: v.cpu().numpy() for k, v in q.items()}
     errs = qoi_errors(qg, q_ref)
+    print("qoi errors (rate floor 1e-9)", errs, flush=True)
+    # against the oracle's converged fixed point (exact Newton to 1e-14) every QoI meets 1e-8
+    # (measured: energy 6e-11, rates 1.1e-9)
     assert max(errs.values()) < 1e-8, errs
     hg = h.cpu().numpy()
     for i in range(B):
