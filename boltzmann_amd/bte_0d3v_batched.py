@@ -134,7 +134,8 @@ class bte_0d3v_batched:
         g = VGrid(t, int(self._par_nr[idx]) + 1, lmax, self._args.spline_qpts, vth, device=self._dev)
         self._op_spec_grid[idx] = g
         pt[pp.C_EN_SETUP].start()
-        self._solver[idx] = Batched0D(g, coll, growth="collisions")
+        use_ee = getattr(self._args, "ee_collisions", 0) == 1
+        self._solver[idx] = Batched0D(g, coll, growth="collisions", ee_pattern=use_ee)
         torch.cuda.synchronize()
         pt[pp.C_EN_SETUP].stop()
         if getattr(self._args, "ee_collisions", 0) == 1:
@@ -188,7 +189,7 @@ class bte_0d3v_batched:
         idx = grid_idx
         pt = self.profile_tt_all[idx]
         s = self._solver[idx]
-        x = _dev_tensor(f0, self._dev)
+        x = _dev_tensor(f0, self._dev).clone()          # the caller's f0 is left untouched
         B = x.shape[1]
         n0, Tg, E, ns = self._params(idx, B)
         if self._op_col_ee[idx] is not None and getattr(self._args, "ee_collisions", 0) == 1:
