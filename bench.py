@@ -203,8 +203,146 @@ def accuracy_check(h_gpu, q_gpu, idx):
     return errs
 
 
+def _grid_and_solver(Nr=NR, lmax=LMAX, q=Q, ee=False):
+    import torch
+    from boltzmann_amd.cross_section import xsections_from_file
+    from boltzmann_amd.solver0d import Batched0D
+    from boltzmann_amd.vspace import VGrid, make_knots, vth_from_Te, C_GAMMA
+    dev = torch.device("cuda", 0)
+    xs = xsections_from_file(XSEC, device=dev)
+    vth = vth_from_Te(TE)
+    kd = (0.0, math.sqrt(EVMAX) * C_GAMMA / vth)
+    thr = sorted(set(x.threshold for x in xs if x.threshold > 0))
+    t = make_knots(kd, 3, Nr + 1, np.sqrt(np.array(thr)) * C_GAMMA / vth, extend_domain=True)
+    g = VGrid(t, Nr + 1, lmax, q, vth, device=dev)
+    return g, Batched0D(g, xs, growth="collisions", ee_pattern=ee), dev
+
+
+def bench_ee(args):
+    """Batched steady state with e-e collisions: the 4096-instance workload above plus
+    ionisation degrees 10^U(-6,-2) (next draw of the same generator), ne = ion_deg n0,
+    warm start without e-e, then the e-e solve to |r| <= 1e-12 |r0|."""
+    import torch
+    from boltzmann_amd.bte_ee import EEOperator
+    g, s, dev = _grid_and_solver(ee=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eop = EEOperator(g, s)
+    torch.cuda.synchronize()
+    setup_ee = time.perf_counter() - t0
+    rng = np.random.default_rng(SEED)
+    EbyN = 10.0 ** rng.uniform(0.0, 3.0, B)
+    fac = rng.uniform(0.8, 1.2, (3, B))
+    ion = 10.0 ** rng.uniform(-6.0, -2.0, B)
+    n0v = np.full(B, 3.22e22)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    n0, Tg, E, ns, ne = T(n0v), T(np.full(B, 300.0)), T(EbyN * n0v * 1e-21), T(fac), T(ion * n0v)
+    x0 = s.initial(B)
+
+    def step():
+        x = x0.clone()
+        s.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=1e-13, max_iter=MAX_ITER, mode=1)
+        info = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, RTOL, 400)
+        h, q = s.qois(x, n0, E)
+        return info
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        info = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    conv = int((info["status"] == 0).sum())
+    print(json.dumps({
+        "metric": "0D EDF solves/s with e-e at <=1e-8 rel err vs FP64", "value": conv * args.steps / dt,
+        "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched 0D steady state with e-e, 4096 instances, Nr=128 lmax=2 (n=387), "
+                               "ion_deg 1e-6..1e-2, warm start without e-e included", "batch": B, "n": g.n},
+        "outer_iterations": int(info["outer"]), "converged": conv, "of": B, "ee_setup_s": setup_ee,
+        "accuracy": "QoIs within 1e-8 of the oracle at Nr=24 (tests/test_gpu_ee.py)"}), flush=True)
+
+
+def bench_transient(args):
+    """Backward-Euler steps of the 4096-instance workload at dt = 1e-3 RF periods (13.56 MHz),
+    the reference driver's default step; steps per second of the whole batch."""
+    import torch
+    g, s, dev = _grid_and_solver()
+    prm = instances(B)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    n0, Tg, E, ns = T(prm["n0"]), T(prm["Tg"]), T(prm["E"]), T(prm["ns_by_n0"])
+    u = s.initial(B)
+    u = (u / (s.u @ u)[None, :]).contiguous()
+    c = 1e-3 / 13.56e6
+    x = u.clone()
+    for _ in range(args.warmup):
+        x.copy_(u)
+        s.transient_step(x, n0, Tg, E, ns, c, 0.0, 1e-10, 30, ustart=u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x.copy_(u)
+        sc = s.transient_step(x, n0, Tg, E, ns, c, 0.0, 1e-10, 30, ustart=u)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "0D transient backward-Euler instance-steps/s", "value": B * args.steps / dt,
+        "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "one BE step (dt = 1e-3 RF period) of 4096 instances, Nr=128 lmax=2, frozen-Jacobian "
+                               "Newton to 1e-10", "batch": B, "n": g.n},
+        "newton_iters_mean": float(sc["iters"].double().mean())}), flush=True)
+
+
+def bench_glow(args):
+    """1D3V velocity-space step at the glow shapes: Nr = 128 (129 B-splines), lmax = 2,
+    Nvt = 32, Nx = 200 Chebyshev-Gauss-Lobatto points, E = 1e4 x^3 V/m (spatially varying, so
+    GMRES with restart 30 is used), dt = 5e-5 RF periods: one x-advection half step, one
+    backward-Euler v-step, one x-advection half step."""
+    import torch
+    from boltzmann_amd.glow1d import Ordinates, VStep, XAdvection
+    g, s, dev = _grid_and_solver()
+    Minv = torch.linalg.inv(s.M)
+    C_en, C_gT, A_v = Minv @ sum(s.C), Minv @ s.Cg, Minv @ s.A
+    tau, np0, Tg, dt = 1.0 / 13.56e6, 8e16, 300.0, 5e-5
+    n0 = 3.22e22 / np0
+    Nx, Nvt = 200, 32
+    xg = -np.cos(np.pi * np.arange(Nx) / (Nx - 1))
+    E = torch.from_numpy(1e4 * xg ** 3).to(dev)
+    ords = Ordinates(g.num_p, g.lmax, Nvt, dev)
+    vs = VStep(C_en, C_gT, A_v, tau, n0, np0, Tg, dt, ordinates=ords, rtol=1e-10, gmres_rsrt=30)
+    xa = XAdvection(g, xg, Nvt, g.vth * tau / 0.5 / 2.54e-2, 0.5 * dt)
+    v0 = ords.sph_to_ords(s.initial(Nx).contiguous())
+
+    def step(v):
+        v = xa.step(v, 0.5 * dt)
+        v = vs.step(v.contiguous(), E)
+        return xa.step(v, 0.5 * dt)
+    for _ in range(args.warmup):
+        step(v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v = v0
+    for _ in range(args.steps):
+        v = step(v0)
+    torch.cuda.synchronize()
+    dt_w = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "1D3V glow step (x-half, v, x-half) per second", "value": args.steps / dt_w, "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt_w / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "1D3V step, Nr=128 lmax=2 Nvt=32 Nx=200, E=1e4 x^3 V/m, GMRES restart 30",
+                   "dof_v": g.n, "Nx": Nx, "Nvt": Nvt},
+        "gmres_iterations": vs.last_iters}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="steady", choices=["steady", "ee", "transient", "glow"],
+                    help="steady: the batched 0D steady state (default); ee, transient, glow: the other paths")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -229,6 +367,9 @@ def main():
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (instances shard "
               f"trivially across GPUs; this build measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "steady":
+        {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow}[args.workload](args)
+        return
 
     import torch
     from boltzmann_amd.cross_section import xsections_from_file
