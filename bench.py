@@ -33,6 +33,7 @@ B = 4096
 NR, LMAX, TE, EVMAX, Q = 128, 2, 4.0, 80.0, 5
 SEED = 20251202
 RTOL = 1e-12
+RTOL_EE = 1e-15        # e-e solves run to the FP64 floor of |F| (status 0, or 3: stopped at the floor)
 MAX_ITER = 12
 RATE_FLOOR = 1e-9
 FLOOR_REL = 1e-11      # a stop with |r| / |r0| below this is at the FP64 roundoff floor
@@ -221,7 +222,7 @@ def _grid_and_solver(Nr=NR, lmax=LMAX, q=Q, ee=False):
 def bench_ee(args):
     """Batched steady state with e-e collisions: the 4096-instance workload above plus
     ionisation degrees 10^U(-6,-2) (next draw of the same generator), ne = ion_deg n0,
-    warm start without e-e, then the e-e solve to |r| <= 1e-12 |r0|."""
+    warm start without e-e, then the e-e Newton-Krylov solve to the FP64 floor of |r|."""
     import torch
     from boltzmann_amd.bte_ee import EEOperator
     g, s, dev = _grid_and_solver(ee=True)
@@ -242,7 +243,7 @@ def bench_ee(args):
     def step():
         x = x0.clone()
         s.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=1e-13, max_iter=MAX_ITER, mode=1)
-        info = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, RTOL, 400)
+        info = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, RTOL_EE, 60)
         h, q = s.qois(x, n0, E)
         return info
     for _ in range(args.warmup):
@@ -253,16 +254,21 @@ def bench_ee(args):
         info = step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    conv = int((info["status"] == 0).sum())
+    st, rel = info["status"], info["res_abs"] / info["res0"]
+    conv = int(((st == 0) | ((st == 3) & (rel <= 1e-12))).sum())
     print(json.dumps({
-        "metric": "0D EDF solves/s with e-e at <=1e-8 rel err vs FP64", "value": conv * args.steps / dt,
+        "metric": "0D EDF solves/s with e-e", "value": conv * args.steps / dt,
         "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched 0D steady state with e-e, 4096 instances, Nr=128 lmax=2 (n=387), "
                                "ion_deg 1e-6..1e-2, warm start without e-e included", "batch": B, "n": g.n},
-        "outer_iterations": int(info["outer"]), "converged": conv, "of": B, "ee_setup_s": setup_ee,
-        "accuracy": "QoIs within 1e-8 of the oracle at Nr=24 (tests/test_gpu_ee.py)"}), flush=True)
+        "rtol": RTOL_EE, "newton_sweeps": int(info["outer"]),
+        "newton_steps_mean": float(info["iters"].double().mean()),
+        "krylov_steps_mean": float(info["krylov"].double().mean()), "converged": conv, "of": B,
+        "ee_setup_s": setup_ee,
+        "accuracy": "QoIs within 1e-8 of the oracle's converged fixed point at Nr=24 (tests/test_gpu_ee.py)"}),
+        flush=True)
 
 
 def bench_transient(args):

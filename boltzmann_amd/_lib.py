@@ -40,7 +40,7 @@ class Batch0D(ctypes.Structure):
 
 class EE0D(ctypes.Structure):
     _fields_ = [("ns", c_int), ("npr", c_int), ("ccA", c_void_p), ("ccB", c_void_p), ("h", c_void_p), ("g", c_void_p),
-                ("negam", c_void_p)]
+                ("negam", c_void_p), ("h_lo", c_void_p), ("g_lo", c_void_p)]
 
 
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;

@@ -10,7 +10,8 @@
 // Per Newton step of the batched solver (the factored form of bte_0d3v_batched.py:444-476,
 // 702-709): potentials h = H x, g = G x of every instance and the Coulomb prefactor
 // ne gamma_a(x) (:652-661); the e-e operator at fixed potentials is then added to the
-// instance's K profile by k_ee_profile_add (bz_solver0d.hip).
+// instance's K profile by k_ee_profile_add (bz_solver0d.hip).  Per Krylov step: the
+// potentials of the direction and the potential-derivative part of the Jacobian (k_ee_t2).
 #include <algorithm>
 
 #include "bz_common.h"
@@ -179,8 +180,10 @@ __global__ void k_cc_banded(const double* __restrict__ t, int npr, int ns, const
 }
 
 // h[b, r*ns + s] = sum_k H[s][r][k] x[(k*ns + s), b]  (and g with G); x (n, B), h, g (B, n)
+// cols = 0: h, g (B, n);  cols = 1: h, g (n, B) like x
 __global__ void k_ee_potentials(const double* __restrict__ H, const double* __restrict__ G, int ns, int npr,
-                                const double* __restrict__ x, int B, double* __restrict__ h, double* __restrict__ g) {
+                                const double* __restrict__ x, int B, int cols, double* __restrict__ h,
+                                double* __restrict__ g) {
   const int n = ns * npr;
   const long total = (long)n * B;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
@@ -194,8 +197,85 @@ __global__ void k_ee_potentials(const double* __restrict__ H, const double* __re
       sh = fma(Hr[k], xv, sh);
       sg = fma(Gr[k], xv, sg);
     }
-    h[(long)b * n + row] = sh;
-    g[(long)b * n + row] = sg;
+    const long o = cols ? e : (long)b * n + row;
+    h[o] = sh;
+    g[o] = sg;
+  }
+}
+
+// s + c += a * b, product and sum errors collected in c (fma TwoProd, Knuth TwoSum)
+__device__ __forceinline__ void dd_acc(double a, double b, double& s, double& c) {
+#pragma clang fp contract(off)   // a contracted s + a*b would lose the product's rounding error
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  const double t = s + p;
+  const double z = t - s;
+  const double se = (s - (t - z)) + (p - z);
+  s = t;
+  c += se + pe;
+}
+
+// h, g (B, n) with the potentials to twice FP64 precision: h + h_lo, g + g_lo
+__global__ void k_ee_potentials_dd(const double* __restrict__ H, const double* __restrict__ G, int ns, int npr,
+                                   const double* __restrict__ x, int B, double* __restrict__ h,
+                                   double* __restrict__ g, double* __restrict__ hlo, double* __restrict__ glo) {
+#pragma clang fp contract(off)
+  const int n = ns * npr;
+  const long total = (long)n * B;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int b = (int)(e % B), row = (int)(e / B);
+    const int r = row / ns, s = row % ns;
+    const double* Hr = H + ((long)s * npr + r) * npr;
+    const double* Gr = G + ((long)s * npr + r) * npr;
+    double sh = 0.0, ch = 0.0, sg = 0.0, cg = 0.0;
+    for (int k = 0; k < npr; ++k) {
+      const double xv = x[(long)(k * ns + s) * B + b];
+      dd_acc(Hr[k], xv, sh, ch);
+      dd_acc(Gr[k], xv, sg, cg);
+    }
+    const long o = (long)b * n + row;
+    const double th = sh + ch, tg = sg + cg;          // renormalise (FastTwoSum)
+    h[o] = th;
+    hlo[o] = ch - (th - sh);
+    g[o] = tg;
+    glo[o] = cg - (tg - sg);
+  }
+}
+
+// The potential-derivative part of the e-e Jacobian (bte_0d3v_batched.py:734-743, the
+// cc_12 x term), in M-space:
+//   t2[b, (p,q)] = negam[b] sum_{(k,l),(r,s)} (cc_a[(p,q),(k,l),(r,s)] hv[(r,s), b]
+//                                              + cc_b[...] gv[(r,s), b]) x[(k,l), b]
+// x, hv, gv (n, B); t2 (B, n).  cc banded: |k - p|, |r - p| <= 3.
+__global__ void k_ee_t2(int ns, int npr, int B, const double* __restrict__ ccA, const double* __restrict__ ccB,
+                        const double* __restrict__ x, const double* __restrict__ hv, const double* __restrict__ gv,
+                        const double* __restrict__ negam, double* __restrict__ t2) {
+  const int n = ns * npr, nc = ns * ns * ns;
+  const long total = (long)n * B;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int inst = (int)(e % B), row = (int)(e / B);
+    const int p = row / ns, q = row - p * ns;
+    double s = 0.0;
+    for (int dk = -3; dk <= 3; ++dk) {
+      const int k = p + dk;
+      if (k < 0 || k >= npr) continue;
+      for (int l = 0; l < ns; ++l) {
+        const double xv = x[(long)(k * ns + l) * B + inst];
+        double a = 0.0;
+        for (int dr = -3; dr <= 3; ++dr) {
+          const int r = p + dr;
+          if (r < 0 || r >= npr) continue;
+          const long base = (((long)p * 7 + (dk + 3)) * 7 + (dr + 3)) * nc + (long)(q * ns + l) * ns;
+          for (int ss = 0; ss < ns; ++ss) {
+            const long pi = (long)(r * ns + ss) * B + inst;
+            a = fma(ccA[base + ss], hv[pi], a);
+            a = fma(ccB[base + ss], gv[pi], a);
+          }
+        }
+        s = fma(a, xv, s);
+      }
+    }
+    t2[(long)inst * n + row] = negam[inst] * s;
   }
 }
 
@@ -219,63 +299,6 @@ __global__ void k_ee_gamma(const double* __restrict__ x, int n, int B, const dou
 }
 
 int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 16384); }
-
-// Anderson mixing of the outer potential iteration, per instance b (columns of (n, B)
-// arrays).  History slots j = 0..m-1 hold dX_j = x_(j+1) - x_j and dF_j = f_(j+1) - f_j
-// (ring buffer, slot stride n*B); f = Phi(x) - x of the current iterate.
-// gamma = argmin |f - dF gamma| by the normal equations (m <= 16, Cholesky with a relative
-// ridge of 1e-13 on the diagonal), x <- x + f - (dX + dF) gamma.
-__global__ void k_anderson(int n, int B, int m, const double* __restrict__ dX, const double* __restrict__ dF,
-                           const double* __restrict__ f, double* __restrict__ x) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double G[16 * 16], r[16];
-  const long slot = (long)n * B;
-  for (int i = 0; i < m; ++i) {
-    double ri = 0.0;
-    for (int e = 0; e < n; ++e) ri = fma(dF[i * slot + (long)e * B + b], f[(long)e * B + b], ri);
-    r[i] = ri;
-    for (int j = 0; j <= i; ++j) {
-      double s = 0.0;
-      for (int e = 0; e < n; ++e) s = fma(dF[i * slot + (long)e * B + b], dF[j * slot + (long)e * B + b], s);
-      G[i * 16 + j] = s;
-      G[j * 16 + i] = s;
-    }
-  }
-  double tr = 0.0;
-  for (int i = 0; i < m; ++i) tr += G[i * 16 + i];
-  const double ridge = 1e-13 * tr + 1e-300;
-  for (int i = 0; i < m; ++i) G[i * 16 + i] += ridge;
-  // Cholesky G = L L^T in place (lower part), then solve
-  for (int j = 0; j < m; ++j) {
-    double d = G[j * 16 + j];
-    for (int k = 0; k < j; ++k) d -= G[j * 16 + k] * G[j * 16 + k];
-    d = sqrt(fmax(d, 1e-300));
-    G[j * 16 + j] = d;
-    for (int i = j + 1; i < m; ++i) {
-      double s = G[i * 16 + j];
-      for (int k = 0; k < j; ++k) s -= G[i * 16 + k] * G[j * 16 + k];
-      G[i * 16 + j] = s / d;
-    }
-  }
-  double gm[16];
-  for (int i = 0; i < m; ++i) {
-    double s = r[i];
-    for (int k = 0; k < i; ++k) s -= G[i * 16 + k] * gm[k];
-    gm[i] = s / G[i * 16 + i];
-  }
-  for (int i = m - 1; i >= 0; --i) {
-    double s = gm[i];
-    for (int k = i + 1; k < m; ++k) s -= G[k * 16 + i] * gm[k];
-    gm[i] = s / G[i * 16 + i];
-  }
-  for (int e = 0; e < n; ++e) {
-    const long o = (long)e * B + b;
-    double v = x[o] + f[o];
-    for (int j = 0; j < m; ++j) v -= gm[j] * (dX[j * slot + o] + dF[j * slot + o]);
-    x[o] = v;
-  }
-}
 
 }  // namespace
 
@@ -323,16 +346,35 @@ int bz_cc_banded(const double* t, int npr, int ns, const double* glx, const doub
 int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h, double* g,
                      hipStream_t stream) {
   if (ns < 1 || npr < 1 || B < 1 || !H || !G || !x || !h || !g) return BZ_EINVAL;
-  hipLaunchKernelGGL(k_ee_potentials, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B, h,
-                     g);
+  hipLaunchKernelGGL(k_ee_potentials, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B, 0,
+                     h, g);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }
 
-int bz_anderson_mix(int n, int B, int m, const double* dX, const double* dF, const double* f, double* x,
-                    hipStream_t stream) {
-  if (n < 1 || B < 1 || m < 0 || m > 16 || !f || !x || (m > 0 && (!dX || !dF))) return BZ_EINVAL;
-  hipLaunchKernelGGL(k_anderson, dim3((B + 63) / 64), dim3(64), 0, stream, n, B, m, dX, dF, f, x);
+int bz_ee_potentials_dd(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h,
+                        double* g, double* h_lo, double* g_lo, hipStream_t stream) {
+  if (ns < 1 || npr < 1 || B < 1 || !H || !G || !x || !h || !g || !h_lo || !g_lo) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_potentials_dd, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B,
+                     h, g, h_lo, g_lo);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_ee_potentials_cols(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h,
+                          double* g, hipStream_t stream) {
+  if (ns < 1 || npr < 1 || B < 1 || !H || !G || !x || !h || !g) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_potentials, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, H, G, ns, npr, x, B, 1,
+                     h, g);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* ccB, const double* x, const double* hv,
+                      const double* gv, const double* negam, double* t2, hipStream_t stream) {
+  if (ns < 1 || npr < 1 || B < 1 || !ccA || !ccB || !x || !hv || !gv || !negam || !t2) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_t2, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, ns, npr, B, ccA, ccB, x, hv,
+                     gv, negam, t2);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

@@ -141,6 +141,107 @@ __global__ void k_vstep_combine(const double* __restrict__ x, const double* __re
     y[e] = x[e] - c * (Kx[e] + E[e % nx] * Ax[e]);
 }
 
+// ---------------------------------------------------------------- batched GMRES (many small systems)
+// B independent systems of size n, one wavefront each.  Basis V[(j * B + b) * n + e],
+// j = 0..m; the Hessenberg columns are reduced by Givens rotations as they arrive, so
+// R[b][col * m + row] holds the triangular factor, g[b][0..m] the rotated right-hand
+// side and |g[b][k]| the residual norm after k steps.
+
+__device__ __forceinline__ double wdot(const double* __restrict__ a, const double* __restrict__ b, int n) {
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n; e += 64) s = fma(a[e], b[e], s);
+  return bz::wave_sum(s);
+}
+
+// Arnoldi step i of every active system: orthogonalise w[b] (the operator applied to
+// the preconditioned V_i) against V_0..V_i (classical Gram-Schmidt, twice), append
+// V_{i+1}, rotate the new column, update the residual estimate; a system whose residual
+// reaches target[b] (or whose basis breaks down) becomes inactive with kdim = i + 1.
+__global__ __launch_bounds__(64) void k_arnoldi_batched(int n, int B, int m, int i, double* __restrict__ V,
+                                                        const double* __restrict__ w, double* __restrict__ R,
+                                                        double* __restrict__ cs, double* __restrict__ sn,
+                                                        double* __restrict__ g, const double* __restrict__ target,
+                                                        int* __restrict__ active, int* __restrict__ kdim,
+                                                        double* __restrict__ resid) {
+  extern __shared__ double lds[];
+  const int inst = blockIdx.x, lane = threadIdx.x;
+  if (!active[inst]) return;
+  double* W = lds;
+  double* HC = W + n;          // [m + 2] Hessenberg column
+  double* T = HC + m + 2;      // [m + 1] projections of one pass
+  for (int e = lane; e < n; e += 64) W[e] = w[(size_t)inst * n + e];
+  for (int j = lane; j <= i + 1; j += 64) HC[j] = 0.0;
+  __syncthreads();
+  for (int rep = 0; rep < 2; ++rep) {
+    for (int j = 0; j <= i; ++j) {
+      const double d = wdot(V + ((size_t)j * B + inst) * n, W, n);
+      if (lane == 0) { T[j] = d; HC[j] += d; }
+    }
+    __syncthreads();
+    for (int e = lane; e < n; e += 64) {
+      double s = W[e];
+      for (int j = 0; j <= i; ++j) s = fma(-T[j], V[((size_t)j * B + inst) * n + e], s);
+      W[e] = s;
+    }
+    __syncthreads();
+  }
+  const double hn = sqrt(wdot(W, W, n));
+  const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
+  double* Vn = V + ((size_t)(i + 1) * B + inst) * n;
+  for (int e = lane; e < n; e += 64) Vn[e] = W[e] * inv;
+  if (lane == 0) {
+    HC[i + 1] = hn;
+    double* csb = cs + (size_t)inst * m;
+    double* snb = sn + (size_t)inst * m;
+    double* gb = g + (size_t)inst * (m + 1);
+    for (int j = 0; j < i; ++j) {
+      const double a = HC[j], b2 = HC[j + 1];
+      HC[j] = fma(csb[j], a, snb[j] * b2);
+      HC[j + 1] = fma(-snb[j], a, csb[j] * b2);
+    }
+    const double a = HC[i], b2 = HC[i + 1];
+    const double r = hypot(a, b2);
+    const double c = r > 0.0 ? a / r : 1.0, s = r > 0.0 ? b2 / r : 0.0;
+    csb[i] = c;
+    snb[i] = s;
+    HC[i] = r;
+    const double gi = gb[i];
+    gb[i] = c * gi;
+    gb[i + 1] = -s * gi;
+    double* Rb = R + (size_t)inst * m * m + (size_t)i * m;
+    for (int j = 0; j <= i; ++j) Rb[j] = HC[j];
+    const double res = fabs(gb[i + 1]);
+    resid[inst] = res;
+    kdim[inst] = i + 1;
+    if (res <= target[inst] || !(hn > 0.0) || i + 1 >= m) active[inst] = 0;
+  }
+}
+
+// out[b] = sum_j y_j V_j[b] with R y = g (the first kdim[b] steps); zero when kdim[b] = 0
+__global__ __launch_bounds__(64) void k_gmres_combine_batched(int n, int B, int m, const double* __restrict__ V,
+                                                              const double* __restrict__ R,
+                                                              const double* __restrict__ g,
+                                                              const int* __restrict__ kdim, double* __restrict__ out) {
+  __shared__ double Y[64];
+  const int inst = blockIdx.x, lane = threadIdx.x;
+  const int k = kdim[inst];
+  if (lane == 0) {
+    const double* Rb = R + (size_t)inst * m * m;
+    const double* gb = g + (size_t)inst * (m + 1);
+    for (int r = k - 1; r >= 0; --r) {
+      double s = gb[r];
+      for (int c = r + 1; c < k; ++c) s = fma(-Rb[(size_t)c * m + r], Y[c], s);
+      Y[r] = Rb[(size_t)r * m + r] != 0.0 ? s / Rb[(size_t)r * m + r] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = lane; e < n; e += 64) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s = fma(Y[j], V[((size_t)j * B + inst) * n + e], s);
+    out[(size_t)inst * n + e] = s;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -200,6 +301,27 @@ int bz_vstep_combine(const double* x, const double* Kx, const double* Ax, const 
 int bz_hess_lstsq(const double* H, int ldh, int m, const double* beta, double* work, double* y, hipStream_t stream) {
   if (m < 1 || ldh < m || !H || !beta || !work || !y) return BZ_EINVAL;
   hipLaunchKernelGGL(k_hess_lstsq, dim3(1), dim3(64), 0, stream, H, ldh, m, beta, work, y);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_arnoldi_batched(int n, int B, int m, int i, double* V, const double* w, double* R, double* cs, double* sn,
+                       double* g, const double* target, int* active, int* kdim, double* resid, hipStream_t stream) {
+  if (n < 1 || B < 1 || m < 1 || m > 64 || i < 0 || i >= m || !V || !w || !R || !cs || !sn || !g || !target ||
+      !active || !kdim || !resid)
+    return BZ_EINVAL;
+  const size_t lds = (size_t)(n + 2 * m + 3) * sizeof(double);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_arnoldi_batched, dim3(B), dim3(64), lds, stream, n, B, m, i, V, w, R, cs, sn, g, target, active,
+                     kdim, resid);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_gmres_combine_batched(int n, int B, int m, const double* V, const double* R, const double* g, const int* kdim,
+                             double* out, hipStream_t stream) {
+  if (n < 1 || B < 1 || m < 1 || m > 64 || !V || !R || !g || !kdim || !out) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_gmres_combine_batched, dim3(B), dim3(64), 0, stream, n, B, m, V, R, g, kdim, out);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

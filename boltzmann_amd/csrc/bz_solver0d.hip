@@ -74,6 +74,19 @@ __device__ __forceinline__ int orig_index(int i, int ns, int npr) {
   return (npr - 1 - q) * ns + l;
 }
 
+// s + c += a * b with the rounding errors of the product (fma) and of the sum (Knuth's
+// TwoSum) collected in c: the result is as accurate as a sum in twice the precision.
+__device__ __forceinline__ void two_sum_acc(double a, double b, double& s, double& c) {
+#pragma clang fp contract(off)   // a contracted s + a*b would lose the product's rounding error
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  const double t = s + p;
+  const double z = t - s;
+  const double se = (s - (t - z)) + (p - z);
+  s = t;
+  c += se + pe;
+}
+
 __device__ __forceinline__ double fast_rcp(double a) {
   double r = __builtin_amdgcn_rcp(a);
   r = fma(r, fma(-a, r, 1.0), r);
@@ -631,6 +644,8 @@ struct EEDev {
   const double* h;      // (B, n) original order
   const double* g;
   const double* negam;  // [B]
+  const double* h_lo;   // (B, n) or null
+  const double* g_lo;
 };
 
 // K profile += ne gamma_a [cc_a(., h) + cc_b(., g)]: the e-e operator of every instance at
@@ -650,18 +665,167 @@ __global__ void k_ee_profile_add(GridDev g, BatchDev b, EEDev e) {
     if (dk < -3 || dk > 3) continue;
     const double* hv = e.h + (size_t)inst * n;
     const double* gv = e.g + (size_t)inst * n;
-    double s = 0.0;
+    const double* hl = e.h_lo ? e.h_lo + (size_t)inst * n : nullptr;
+    const double* gl = e.g_lo ? e.g_lo + (size_t)inst * n : nullptr;
+    // Compensated sum: near v_max the products cancel to ~1e-6 of their size, so a plain
+    // FP64 sum carries order-dependent errors that bias the e-e fixed point.
+    double s = 0.0, c = 0.0;
     for (int dr = -3; dr <= 3; ++dr) {
       const int r = p + dr;
       if (r < 0 || r >= npr) continue;
       const size_t base = (((size_t)p * 7 + (dk + 3)) * 7 + (dr + 3)) * nc + (size_t)(q * ns + l) * ns;
       for (int ss = 0; ss < ns; ++ss) {
-        s = fma(e.ccA[base + ss], hv[r * ns + ss], s);
-        s = fma(e.ccB[base + ss], gv[r * ns + ss], s);
+        two_sum_acc(e.ccA[base + ss], hv[r * ns + ss], s, c);
+        two_sum_acc(e.ccB[base + ss], gv[r * ns + ss], s, c);
+        if (hl) c = fma(e.ccA[base + ss], hl[r * ns + ss], fma(e.ccB[base + ss], gl[r * ns + ss], c));
       }
     }
-    b.kprof[idx] = fma(e.negam[inst], s, b.kprof[idx]);
+    b.kprof[idx] = fma(e.negam[inst], s + c, b.kprof[idx]);
   }
+}
+
+// ---------------------------------------------------------------- e-e Newton-Krylov pieces
+// The Newton system with e-e, J d = -F with
+//   F(x)  = P (M^-1 K_x x - mu x),      K_x = K + ne gamma_a [cc_a(., Hx) + cc_b(., Gx)]
+//   J v   = P (M^-1 (K_x v + T2 v) - mu v - x (w.v)),   T2 v = ne gamma_a [cc_a(x, Hv) + cc_b(x, Gv)]
+// (P = I - uhat uhat^T; the reference's Jacobian, bte_0d3v_batched.py:734-743, plus the
+// growth derivative) is solved by GMRES on the host with the banded part as right
+// preconditioner; these kernels are its building blocks.  The instance profile holds
+// K_x - mu M (mu = w.x of the Newton iterate), one wavefront per instance.
+
+// mu[b] = w.x_b and the profile shift K_x -> K_x - mu M
+__global__ __launch_bounds__(64) void k_ee_mu_shift(GridDev g, BatchDev b, double* __restrict__ mu_out) {
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  const double* WV = b.vec + (size_t)inst * 4 * n;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) s = fma(WV[i], b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst], s);
+  const double mu = wave_sum(s);
+  if (lane == 0) mu_out[inst] = mu;
+  double* kprof = b.kprof + (size_t)inst * g.P;
+  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+  for (int k = lane; k < 7 * n; k += 64) {
+    const int i = k / 7, dd = k - 7 * (k / 7) - 3;
+    const int c = i + dd * g.ns;
+    if (c >= 0 && c < n) {
+      const int e = g.prof_off[i] + c - g.first_col[i];
+      kprof[e] = fma(-mu, mprof[e], kprof[e]);
+    }
+  }
+}
+
+// Stage the mass Cholesky rows (LCH, 4*npr doubles) used by mass_solve.
+__device__ void stage_mass(const GridDev& g, double* LCH, int lane) {
+  for (int e = lane; e < 4 * g.npr; e += 64) {
+    const int q = e >> 2, dd = e & 3;
+    const double v = g.mchol[(size_t)(q * g.ns) * 4 + dd];
+    LCH[e] = (dd == 0) ? 1.0 / v : ((q - dd >= 0) ? v : 0.0);
+  }
+}
+
+// mode 0: out = F(x), nrm[b] = |F|;  mode 1: out = J v (v, t2 given, (B, n) original order).
+// Instances with active[b] == 0 (when active is given) are skipped.
+__global__ __launch_bounds__(64) void k_ee_apply(GridDev g, BatchDev b, int mode, const int* __restrict__ active,
+                                                 const double* __restrict__ v, const double* __restrict__ t2,
+                                                 double* __restrict__ out, double* __restrict__ nrm) {
+  extern __shared__ double lds[];
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  double* X = lds;
+  double* V = X + n;
+  double* AV = V + n;
+  double* LCH = AV + n;
+  const double* kprof = b.kprof + (size_t)inst * g.P;
+  const double* WV = b.vec + (size_t)inst * 4 * n;
+  const double* UH = g.uhat;
+  for (int i = lane; i < n; i += 64) {
+    const int o = orig_index(i, g.ns, g.npr);
+    X[i] = b.x[(size_t)o * b.B + inst];
+    V[i] = mode ? v[(size_t)inst * n + o] : X[i];
+    AV[i] = mode ? t2[(size_t)inst * n + o] : 0.0;
+  }
+  stage_mass(g, LCH, lane);
+  __syncthreads();
+  flat_matvec(g.P, kprof, g.ent_rc, V, AV, lane);                 // (K_x - mu M) v (+ T2 v)
+  __syncthreads();
+  mass_solve(g.ns, g.npr, LCH, AV, lane);                         // M^-1 (...)
+  __syncthreads();
+  double wv = 0.0;
+  if (mode) wv = dot(WV, V, n, lane);
+  double su = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double r = mode ? fma(-wv, X[i], AV[i]) : AV[i];
+    AV[i] = r;
+    su = fma(UH[i], r, su);
+  }
+  su = wave_sum(su);
+  double s2 = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double r = AV[i] - su * UH[i];
+    out[(size_t)inst * n + orig_index(i, g.ns, g.npr)] = r;
+    s2 = fma(r, r, s2);
+  }
+  s2 = wave_sum(s2);
+  if (lane == 0 && nrm) nrm[inst] = sqrt(s2);
+}
+
+// Right preconditioner: z in uhat-perp with P (M^-1 (K_x - mu M) z) = v, i.e.
+// (K_x - mu M) z = M (v + c uhat) with c such that uhat.z = 0 (two right-hand sides).
+// v, out (B, n) original order; outT (n, B) the same values, when given.
+template <int W, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_ee_precond(
+    GridDev g, BatchDev b, const int* __restrict__ active, const double* __restrict__ v, double* __restrict__ out,
+    double* __restrict__ outT) {
+  extern __shared__ double lds[];
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  constexpr int WP = (W + 1) & ~1;
+  constexpr int PIVN = (W + 3) & ~1;
+  const int nchunk = (n + W - 1) / W;
+  double* VV = lds;
+  double* Z1 = VV + n;
+  double* Z2 = Z1 + n;
+  double* AV = Z2 + n + zpad(n, g.npr);
+  double* STG = AV + n;
+  double* PIV = STG + 2 * W * WP;
+  int* PO = reinterpret_cast<int*>(PIV + PIVN);
+  short* FC = reinterpret_cast<short*>(PO + (n + 1));
+  short* NX = FC + n;
+  short* LEV = NX + n;
+  const double* kprof = b.kprof + (size_t)inst * g.P;
+  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+  double* urows = b.urows + (size_t)inst * n * WP;
+  for (int i = lane; i < n; i += 64) {
+    FC[i] = (short)g.first_col[i];
+    NX[i] = (short)g.sched_next[i];
+    VV[i] = v[(size_t)inst * n + orig_index(i, g.ns, g.npr)];
+    Z1[i] = g.muhat[i];
+  }
+  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
+  for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
+  __syncthreads();
+  mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, VV, Z2, lane);
+  __syncthreads();
+  factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
+  __syncthreads();
+  back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
+  __syncthreads();
+  const double c = -dot(g.uhat, Z2, n, lane) / dot(g.uhat, Z1, n, lane);
+  for (int i = lane; i < n; i += 64) {
+    const int o = orig_index(i, g.ns, g.npr);
+    const double z = fma(c, Z1[i], Z2[i]);
+    out[(size_t)inst * n + o] = z;
+    if (outT) outT[(size_t)o * b.B + inst] = z;
+  }
+}
+
+template <int W, int R>
+int launch_ee_precond(const GridDev& g, const BatchDev& b, const int* active, const double* v, double* out,
+                      double* outT, hipStream_t stream) {
+  const size_t lds = newton_lds_bytes<W>(g.n, g.npr);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL((k_ee_precond<W, R>), dim3(b.B), dim3(64), lds, stream, g, b, active, v, out, outT);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
 }
 
 template <int W, int R>
@@ -1032,8 +1196,72 @@ int bz_steady_solve_0d_ee(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const b
   BatchDev b;
   if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
   if (ep->ns != g.ns || ep->npr != g.npr || !ep->ccA || !ep->ccB || !ep->h || !ep->g || !ep->negam) return BZ_EINVAL;
-  EEDev e{ep->ccA, ep->ccB, ep->h, ep->g, ep->negam};
+  if ((ep->h_lo == nullptr) != (ep->g_lo == nullptr)) return BZ_EINVAL;
+  EEDev e{ep->ccA, ep->ccB, ep->h, ep->g, ep->negam, ep->h_lo, ep->g_lo};
   return steady_dispatch(g, b, stream, &e);
+}
+
+int bz_ee_newton_residual(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const bz_ee0d_t* ep, double* mu, double* F,
+                          double* fnorm, hipStream_t stream) {
+  if (!gp || !bp || !ep || !mu || !F || !fnorm) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  if (ep->ns != g.ns || ep->npr != g.npr || !ep->ccA || !ep->ccB || !ep->h || !ep->g || !ep->negam) return BZ_EINVAL;
+  if ((ep->h_lo == nullptr) != (ep->g_lo == nullptr)) return BZ_EINVAL;
+  EEDev e{ep->ccA, ep->ccB, ep->h, ep->g, ep->negam, ep->h_lo, ep->g_lo};
+  const long work = (long)b.B * g.P;
+  const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_prepare0d, dim3(blocks), dim3(256), 0, stream, g, b);
+  BZ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_ee_profile_add, dim3(blocks), dim3(256), 0, stream, g, b, e);
+  BZ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_ee_mu_shift, dim3(b.B), dim3(64), 0, stream, g, b, mu);
+  BZ_LAUNCH_CHECK();
+  const size_t lds = (size_t)(3 * g.n + 4 * g.npr) * sizeof(double);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_apply, dim3(b.B), dim3(64), lds, stream, g, b, 0, (const int*)nullptr,
+                     (const double*)nullptr, (const double*)nullptr, F, fnorm);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_ee_newton_precond(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, const double* v, double* z,
+                         double* zT, hipStream_t stream) {
+  if (!gp || !bp || !v || !z) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  const int Wneed = g.bw + 1;
+  const int R = g.nslots / 64;
+  if (Wneed <= 8) {
+    if (R == 1) return launch_ee_precond<8, 1>(g, b, active, v, z, zT, stream);
+    if (R == 2) return launch_ee_precond<8, 2>(g, b, active, v, z, zT, stream);
+    if (R == 3) return launch_ee_precond<8, 3>(g, b, active, v, z, zT, stream);
+  } else if (Wneed <= 11) {
+    if (R == 1) return launch_ee_precond<11, 1>(g, b, active, v, z, zT, stream);
+    if (R == 2) return launch_ee_precond<11, 2>(g, b, active, v, z, zT, stream);
+    if (R == 3) return launch_ee_precond<11, 3>(g, b, active, v, z, zT, stream);
+  } else if (Wneed <= 17) {
+    if (R == 1) return launch_ee_precond<17, 1>(g, b, active, v, z, zT, stream);
+    if (R == 2) return launch_ee_precond<17, 2>(g, b, active, v, z, zT, stream);
+    if (R == 3) return launch_ee_precond<17, 3>(g, b, active, v, z, zT, stream);
+    if (R == 4) return launch_ee_precond<17, 4>(g, b, active, v, z, zT, stream);
+  }
+  return BZ_EINVAL;
+}
+
+int bz_ee_newton_jv(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, const double* z,
+                    const double* t2, double* out, hipStream_t stream) {
+  if (!gp || !bp || !z || !t2 || !out) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  const size_t lds = (size_t)(3 * g.n + 4 * g.npr) * sizeof(double);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_ee_apply, dim3(b.B), dim3(64), lds, stream, g, b, 1, active, z, t2, out, (double*)nullptr);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
 }
 
 int bz_solver0d_window(int bw) {

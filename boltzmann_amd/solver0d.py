@@ -79,11 +79,10 @@ class Batched0D:
                 res0=torch.empty(B, dtype=torch.float64, device=dev))}
         return self._scratch[B]
 
-    def solve_raw(self, x, n0, Tg, E, ns_by_n0, atol=1e-10, rtol=1e-10, max_iter=30, alpha_min=1e-16,
-                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None, tol_abs=None, ee=None):
-        """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0).
-        tol_abs: optional (B,) per-instance absolute tolerance; ee: optional _lib.EE0D with the
-        frozen e-e operator of every instance (bz_steady_solve_0d_ee)."""
+    def batch_struct(self, x, n0, Tg, E, ns_by_n0, atol=0.0, rtol=0.0, max_iter=0, alpha_min=1e-16, alpha_rho=0.5,
+                     ls_c=1e-4, mode=1, tg_scale=1.0, tol_abs=None):
+        """bz_batch0d_t over x (n, B) and the per-instance parameters, with this solver's
+        scratch for B instances; returns (struct, scratch dict)."""
         B = x.shape[1]
         assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
         for v in (n0, Tg, E):
@@ -100,6 +99,16 @@ class Batched0D:
                           ls_c=float(ls_c), max_iter=int(max_iter), mode=int(mode), tg_scale=float(tg_scale),
                           tol_abs=None if tol_abs is None else tol_abs.data_ptr())
         self._last_batch = bt
+        return bt, sc
+
+    def solve_raw(self, x, n0, Tg, E, ns_by_n0, atol=1e-10, rtol=1e-10, max_iter=30, alpha_min=1e-16,
+                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None, tol_abs=None, ee=None):
+        """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0).
+        tol_abs: optional (B,) per-instance absolute tolerance; ee: optional _lib.EE0D with the
+        frozen e-e operator of every instance (bz_steady_solve_0d_ee)."""
+        bt, sc = self.batch_struct(x, n0, Tg, E, ns_by_n0, atol=atol, rtol=rtol, max_iter=max_iter,
+                                   alpha_min=alpha_min, alpha_rho=alpha_rho, ls_c=ls_c, mode=mode,
+                                   tg_scale=tg_scale, tol_abs=tol_abs)
         if ee is None:
             _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
         else:

@@ -147,6 +147,8 @@ typedef struct {
   const double* h;        /* (B, n) first Rosenbluth potential of every instance       */
   const double* g;        /* (B, n) second potential                                   */
   const double* negam;    /* [B]   ne * gamma_a                                         */
+  const double* h_lo;     /* (B, n) low parts of h (h + h_lo in twice FP64 precision), or NULL */
+  const double* g_lo;     /* (B, n) low parts of g, or NULL                              */
 } bz_ee0d_t;
 
 /* bz_steady_solve_0d with the e-e operator at the given (frozen) potentials added to every
@@ -156,6 +158,27 @@ typedef struct {
  * The grid tables must be planned with the e-e pattern (|k - p| <= 3 for all l, q). */
 int bz_steady_solve_0d_ee(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const bz_ee0d_t* ee,
                           hipStream_t stream);
+
+/* Newton-Krylov pieces of the e-e steady state (bte_0d3v_batched.py:702-743, 882-979: the
+ * residual and the Jacobian with the potential derivative cc x + cc_12 x, plus the growth
+ * derivative).  With mu = w.x and K_x = K + ne gamma_a [cc_a(., Hx) + cc_b(., Gx)]:
+ *   F(x) = P (M^-1 K_x x - mu x),   J v = P (M^-1 (K_x v + T2 v) - mu v - x (w.v)),
+ * P = I - uhat uhat^T.  x (n, B) in batch->x (normalised, u.x = 1).
+ *
+ * bz_ee_newton_residual: builds the profiles K_x - mu M in batch->kprof from the potentials
+ * in ee (of x), mu[b], F (B, n), fnorm[b] = |F_b|. */
+int bz_ee_newton_residual(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const bz_ee0d_t* ee, double* mu,
+                          double* F, double* fnorm, hipStream_t stream);
+
+/* Right preconditioner of the Newton system: z (B, n) orthogonal to uhat with
+ * P M^-1 (K_x - mu M) z = v, from the profiles left by bz_ee_newton_residual; zT (n, B),
+ * optional, the same values.  Systems with active[b] == 0 are skipped (active optional). */
+int bz_ee_newton_precond(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, const double* v,
+                         double* z, double* zT, hipStream_t stream);
+
+/* out (B, n) = J z with t2 (B, n) = T2 z from bz_ee_jacobian_t2. */
+int bz_ee_newton_jv(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, const double* z,
+                    const double* t2, double* out, hipStream_t stream);
 
 /* Window width (columns) the solver uses for an upper bandwidth bw (-1: unsupported). */
 int bz_solver0d_window(int bw);
@@ -259,6 +282,19 @@ int bz_scale_inv(const double* x, const double* s, double* y, long n, hipStream_
  * xp.linalg.lstsq call of utils.gmres_pcr (utils.py:1228). */
 int bz_hess_lstsq(const double* H, int ldh, int m, const double* beta, double* work, double* y, hipStream_t stream);
 
+/* GMRES over B independent systems of size n (one per 0D instance), basis
+ * V[(j*B + b)*n + e], j = 0..m (m <= 64).  Arnoldi step i: w (B, n) = A M^-1 V_i; classical
+ * Gram-Schmidt twice, V_{i+1}, the Givens-rotated column into R[b][i*m + 0..i], cs, sn
+ * ([B][m]), g ([B][m+1], g[b][0] = |r0| set by the caller), resid[b] = |g[b][i+1]|;
+ * a system stops (active[b] = 0, kdim[b] = i + 1) at resid <= target[b], breakdown or
+ * i + 1 = m.  The batched form of the Arnoldi loop of utils.gmres_pcr (utils.py:1210-1240). */
+int bz_arnoldi_batched(int n, int B, int m, int i, double* V, const double* w, double* R, double* cs, double* sn,
+                       double* g, const double* target, int* active, int* kdim, double* resid, hipStream_t stream);
+
+/* out (B, n) = sum_j y_j V_j with R y = g over the first kdim[b] steps (zero if kdim[b] = 0). */
+int bz_gmres_combine_batched(int n, int B, int m, const double* V, const double* R, const double* g, const int* kdim,
+                             double* out, hipStream_t stream);
+
 /* ------------------------------------------------------------------ 1D3V v-space step */
 
 /* bin[x] = argmin_i |E[x] - Ev[i]| (first minimum): the preconditioner membership of
@@ -308,18 +344,27 @@ int bz_cc_banded(const double* t, int npr, int ns, const double* glx, const doub
 int bz_ee_potentials(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h, double* g,
                      hipStream_t stream);
 
+/* bz_ee_potentials with the sums compensated: h + h_lo, g + g_lo carry the potentials to
+ * twice FP64 precision (the e-e operator entries near v_max cancel to ~1e-6 of their terms,
+ * so the rounding of plain FP64 potentials moves the fixed point). */
+int bz_ee_potentials_dd(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h,
+                        double* g, double* h_lo, double* g_lo, hipStream_t stream);
+
+/* bz_ee_potentials with h, g in the layout of x, (n, B). */
+int bz_ee_potentials_cols(const double* H, const double* G, int ns, int npr, const double* x, int B, double* h,
+                          double* g, hipStream_t stream);
+
+/* Potential-derivative term of the e-e Jacobian (the cc_12 x product of
+ * bte_0d3v_batched.py:734-743) in M-space: t2[b, (p,q)] = negam[b] sum cc_a[(p,q),(k,l),(r,s)]
+ * hv[(r,s), b] x[(k,l), b] + (cc_b, gv); x, hv, gv (n, B) (hv, gv the potentials of the
+ * direction), t2 (B, n). */
+int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* ccB, const double* x, const double* hv,
+                      const double* gv, const double* negam, double* t2, hipStream_t stream);
+
 /* negam[b] = ne[b] gamma_a(x_b): m0 = c_m0 mass.x, kT = |c_kt top.x / m0|,
  * Lambda = c_lam kT^1.5 / sqrt(ne), gamma_a = ln(Lambda) c_gam (bte_0d3v_batched.py:652-661). */
 int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double* top, const double* ne, double c_m0,
                 double c_kt, double c_lam, double c_gam, double* negam, hipStream_t stream);
-
-/* Anderson mixing step of the outer e-e potential iteration for every instance (columns of
- * (n, B) arrays): gamma = argmin |f - dF gamma| over the m <= 16 history differences
- * (dX, dF: m slots of (n, B)), x <- x + f - (dX + dF) gamma.  No reference counterpart: the
- * reference differentiates the potentials inside its dense Jacobian
- * (bte_0d3v_batched.py:734-743); this accelerates the banded outer iteration that replaces it. */
-int bz_anderson_mix(int n, int B, int m, const double* dX, const double* dF, const double* f, double* x,
-                    hipStream_t stream);
 
 #ifdef __cplusplus
 }

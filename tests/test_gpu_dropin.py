@@ -63,7 +63,7 @@ def test_dropin_ee_with_warm_start():
     args.ee_collisions = 0
     ff, _ = bte.solve(0, f0, 0.0, 1e-13, 40, "steady-state")
     args.ee_collisions = 1
-    h, q = bte.solve(0, ff, 0.0, 1e-12, 400, "steady-state")
+    h, q = bte.solve(0, ff, 0.0, 1e-15, 60, "steady-state")
     ops, g, procs = oracle_batched(24, 1)
     from boltzmann_amd.ee_terms_table import TERMS
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)

@@ -92,17 +92,94 @@ def test_steady_state_with_ee(setup):
         ops.ee = None
     x = _d(x0)
     n0, Tg, E, ns, ne = (_d(prm[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
-    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-12, 400)
+    # solved to the FP64 floor: the tail rates need |F| ~ 1e-15 |F0| (a residual of 4e-14 |F0|
+    # left 1e-8 on a rate at 2e-6 of the largest)
+    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-15, 60)
     torch.cuda.synchronize()
-    print("outer iterations", out["outer"], "inner", out["iters"].cpu().numpy(), flush=True)
-    assert (out["status"].cpu().numpy() == 0).all()
+    st, ra, r0 = (out[k].cpu().numpy() for k in ("status", "res_abs", "res0"))
+    print("newton sweeps", out["outer"], "steps", out["iters"].cpu().numpy(), "krylov",
+          out["krylov"].cpu().numpy(), "status", st, "|F|/|F0|", ra / r0, flush=True)
+    # converged, or stopped at the FP64 evaluation floor of |F| (measured 1e-14 |F0|)
+    assert np.all((st == 0) | ((st == 3) & (ra <= 1e-12 * r0)))
     h, q = s.qois(x, n0, E)
     qg = {k: v.cpu().numpy() for k, v in q.items()}
     errs = qoi_errors(qg, q_ref)
     print("qoi errors (rate floor 1e-9)", errs, flush=True)
     # against the oracle's converged fixed point (exact Newton to 1e-14) every QoI meets 1e-8
-    # (measured: energy 6e-11, rates 1.1e-9)
+    # (measured: energy 6.4e-11, rates 9.5e-10, the spread of the operator assembly: the
+    # oracle's fixed point with this library's H, G, cc_a, cc_b evaluated in long double is
+    # 9.7e-10 away on the same rate)
     assert max(errs.values()) < 1e-8, errs
     hg = h.cpu().numpy()
     for i in range(B):
         assert rel_err(hg[:, i], x_ref[:, i]) < 1e-8
+
+
+def test_newton_krylov_pieces(setup):
+    """Residual, Jacobian-vector product and preconditioner of the e-e Newton-Krylov solve
+    against the oracle's dense residual and Jacobian (cc x + cc_12 x, growth derivative)."""
+    from boltzmann_amd import _lib
+    from boltzmann_amd.bte_ee import _NKWork
+    ops, g, s, eop, hl, gl, A, Bt = setup
+    print("case: e-e Newton-Krylov pieces Nr=24 lmax=1 B=3", flush=True)
+    B = 3
+    n = g.num_p * g.num_sh
+    prm = instance_params(B, seed=5)
+    prm["ne"] = prm["n0"] * np.array([1e-2, 1e-3, 1e-4])
+    x0, _, _ = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, 1e-13, 40, jacobian="exact")
+    x0 = x0 / (ops.u @ x0)[None, :]
+    cc = eeo.ee_operator(ops.Minv, A, Bt, hl, gl)
+    ops.ee = {"cc": cc}
+    try:
+        res, jac = s0.batched_res_jac(ops, prm, jacobian="exact")
+        r_ref = res(x0)
+        J_ref = jac(x0)
+    finally:
+        ops.ee = None
+    Q = ops.Q
+    x = _d(x0)
+    n0, Tg, E, ns, ne = (_d(prm[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
+    w = _NKWork(n, B, 4, x.device)
+    h, gg, negam = eop.potentials(x, ne)
+    ee = _lib.EE0D(ns=eop.ns, npr=eop.npr, ccA=eop.ccA.data_ptr(), ccB=eop.ccB.data_ptr(), h=h.data_ptr(),
+                   g=gg.data_ptr(), negam=negam.data_ptr())
+    bt, _ = s.batch_struct(x, n0, Tg, E, ns)
+    grid = s.tables.struct
+    _lib.call("bz_ee_newton_residual", grid, bt, ee, w.mu, w.F, w.fn)
+    F = w.F.cpu().numpy().T
+    # the e-e term is evaluated in two association orders (dense M^-1 cc on the oracle side,
+    # banded M-space sums then M^-1 here): agreement to ~1e-8 of |F| at the warm start
+    scale = np.abs(r_ref).max(axis=0)
+    print("residual err", np.abs(Q.T @ F - r_ref).max(axis=0) / scale, flush=True)
+    assert np.all(np.abs(Q.T @ F - r_ref).max(axis=0) <= 1e-7 * scale)
+    np.testing.assert_allclose(w.fn.cpu().numpy(), np.linalg.norm(r_ref, axis=0), rtol=1e-7)
+    # J v for v in u-perp
+    rng = np.random.default_rng(7)
+    y = rng.standard_normal((n - 1, B))
+    v = Q @ y
+    w.z.copy_(_d(v.T))
+    w.zT.copy_(_d(v))
+    _lib.call("bz_ee_potentials_cols", eop.H, eop.G, eop.ns, eop.npr, w.zT, B, w.hz, w.gz)
+    _lib.call("bz_ee_jacobian_t2", eop.ns, eop.npr, B, eop.ccA, eop.ccB, x, w.hz, w.gz, negam, w.t2)
+    _lib.call("bz_ee_newton_jv", grid, bt, None, w.z, w.t2, w.Jz)
+    jv = Q.T @ w.Jz.cpu().numpy().T
+    jv_ref = np.einsum("xab,bx->ax", J_ref, y)
+    err = np.abs(jv - jv_ref).max(axis=0) / np.abs(jv_ref).max(axis=0)
+    print("J v err", err, flush=True)
+    assert np.all(err <= 1e-7)
+    # preconditioner: z in u-perp with Q^T (L_x - mu) z = Q^T v
+    _lib.call("bz_ee_newton_precond", grid, bt, None, w.z, w.Jz, None)
+    z = w.Jz.cpu().numpy().T
+    nsb = prm["n0"] * prm["ns_by_n0"]
+    gam = eeo.gamma_a(ops, x0, prm["ne"])
+    for b in range(B):
+        L = (np.einsum("s,suv->uv", nsb[:, b], ops.c_en) + prm["n0"][b] * prm["Tg"][b] * ops.c_gT
+             + prm["E"][b] * ops.adv + prm["ne"][b] * gam[b] * np.tensordot(cc, x0[:, b], axes=([2], [0])))
+        mu = nsb[:, b] @ (ops.W @ x0[:, b])
+        lhs = Q.T @ (L @ z[:, b] - mu * z[:, b])
+        assert abs(ops.u @ z[:, b]) <= 1e-12 * np.abs(ops.u).max() * np.abs(z[:, b]).max()
+        e = np.abs(lhs - y[:, b]).max() / np.abs(y[:, b]).max()
+        print("precond err", b, e, flush=True)
+        # a solve with K_x - mu M (condition ~1e8 here); only the Krylov convergence rate
+        # depends on it
+        assert e <= 1e-7
