@@ -266,6 +266,7 @@ def bench_ee(args):
         "rtol": RTOL_EE, "newton_sweeps": int(info["outer"]),
         "newton_steps_mean": float(info["iters"].double().mean()),
         "krylov_steps_mean": float(info["krylov"].double().mean()), "converged": conv, "of": B,
+        "status_counts": {str(k): int((st == k).sum()) for k in range(4)},
         "ee_setup_s": setup_ee,
         "accuracy": "QoIs within 1e-8 of the oracle's converged fixed point at Nr=24 (tests/test_gpu_ee.py)"}),
         flush=True)

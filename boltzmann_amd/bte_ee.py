@@ -143,9 +143,22 @@ class EEOperator:
                   self.c_gam, negam)
         return h, g, negam, h_lo, g_lo
 
+    def residual_norm(self, x, prm, tg_scale=1.0):
+        """|F(x)| (B,) of the Newton residual with e-e at x (n, B); prm = (n0, Tg, E, ns, ne)."""
+        B = x.shape[1]
+        h, g, negam, h_lo, g_lo = self.potentials_dd(x, prm[4])
+        ee = _lib.EE0D(ns=self.ns, npr=self.npr, ccA=self.ccA.data_ptr(), ccB=self.ccB.data_ptr(), h=h.data_ptr(),
+                       g=g.data_ptr(), negam=negam.data_ptr(), h_lo=h_lo.data_ptr(), g_lo=g_lo.data_ptr())
+        bt, _ = self.s.batch_struct(x, *prm[:4], tg_scale=tg_scale)
+        mu = torch.empty(B, dtype=torch.float64, device=self.dev)
+        F = torch.empty((B, self.n), dtype=torch.float64, device=self.dev)
+        fn = torch.empty_like(mu)
+        _lib.call("bz_ee_newton_residual", self.s.tables.struct, bt, ee, mu, F, fn)
+        return fn
+
     def steady_state(self, x, n0, Tg, E, ns, ne, atol, rtol, max_iter, alpha_min=1e-16, alpha_rho=0.5,
                      line_search_c=1e-4, restart=30, forcing=1e-10, tg_scale=1.0, callback=None,
-                     floor_rel=1e-8, floor_alpha=1e-6):
+                     floor_rel=1e-8, floor_alpha=1.0 / 64):
         """Newton-Krylov on every instance, in place on x (n, B) (normalised to u.x = 1 first,
         as the reference does).  Stops an instance at |F| <= atol + rtol |F(x0)|.  Returns a
         dict with per-instance iters (Newton steps), krylov (GMRES steps), status (0 converged,
@@ -157,7 +170,8 @@ class EEOperator:
         n = 50: 1e-12 r0 to 8e-12 r0).  An instance whose |F| is below floor_rel r0 and whose
         line search finds no decrease down to a step of floor_alpha stops there (status 3);
         the reference would iterate on to max_iter at that point.
-        callback(sweep, |F| (B,), step lengths (B,), Krylov steps (B,)) after every sweep.
+        callback(sweep, instance indices, |F|, step lengths, Krylov steps) over the working set
+        after every sweep.
 
         forcing: each Newton system is solved to |J d + F| <= max(forcing |F|, tol / 4).  The
         Jacobian is ill-conditioned (measured at n = 50: a linear residual of 1e-4 |F| leaves
@@ -165,95 +179,132 @@ class EEOperator:
         steps reach it with the frozen-potential preconditioner."""
         B = x.shape[1]
         s = self.s
-        n, dev, f64 = self.n, self.dev, torch.float64
+        n, dev, f64, i32 = self.n, self.dev, torch.float64, torch.int32
         m = int(restart)
         x.div_((s.u @ x)[None, :])
+        grid = s.tables.struct
+        iters = torch.zeros(B, dtype=i32, device=dev)
+        krylov = torch.zeros(B, dtype=i32, device=dev)
+        failed = torch.zeros(B, dtype=torch.bool, device=dev)
+        floor = torch.zeros(B, dtype=torch.bool, device=dev)
+        res = torch.empty(B, dtype=f64, device=dev)
+
+        # Working set: the instances still iterating, gathered into a dense sub-batch whenever
+        # fewer than half of the current one remain, so that the slow tail of a batch does not
+        # pay for whole-batch sweeps.
+        idx = torch.arange(B, device=dev)
+        xw, prm = x, (n0, Tg, E, ns, ne)
         w = self._ws.get((B, m))
         if w is None:
             self._ws = {(B, m): _NKWork(n, B, m, dev)}
             w = self._ws[(B, m)]
-        grid = s.tables.struct
-        iters = torch.zeros(B, dtype=torch.int32, device=dev)
-        krylov = torch.zeros(B, dtype=torch.int32, device=dev)
-        status = torch.zeros(B, dtype=torch.int32, device=dev)
-        failed = torch.zeros(B, dtype=torch.bool, device=dev)
-        floor = torch.zeros(B, dtype=torch.bool, device=dev)
-        x_try = torch.empty_like(x)
 
-        def evaluate(xe):
+        def evaluate(xe, w, prm):
             """potentials, profiles K_x - mu M, F and |F| at xe; returns the batch struct and negam"""
-            h, g, negam, h_lo, g_lo = self.potentials_dd(xe, ne)
+            h, g, negam, h_lo, g_lo = self.potentials_dd(xe, prm[4])
             ee = _lib.EE0D(ns=self.ns, npr=self.npr, ccA=self.ccA.data_ptr(), ccB=self.ccB.data_ptr(),
                            h=h.data_ptr(), g=g.data_ptr(), negam=negam.data_ptr(), h_lo=h_lo.data_ptr(),
                            g_lo=g_lo.data_ptr())
-            bt, _ = s.batch_struct(xe, n0, Tg, E, ns, tg_scale=tg_scale)
+            bt, _ = s.batch_struct(xe, *prm[:4], tg_scale=tg_scale)
             _lib.call("bz_ee_newton_residual", grid, bt, ee, w.mu, w.F, w.fn)
             return bt, negam, (h, g, h_lo, g_lo)
 
-        bt, negam, keep = evaluate(x)
+        bt, negam, keep = evaluate(xw, w, prm)
         r0 = w.fn.clone()
         tol = atol + rtol * r0
+        tolw = tol
+        x_try = torch.empty_like(xw)
         sweeps = 0
         while True:
-            todo = (w.fn > tol) & ~failed & ~floor
-            if sweeps >= max_iter or not bool(todo.any()):
+            Bw = xw.shape[1]
+            todo = (w.fn > tolw) & ~failed[idx] & ~floor[idx]
+            if sweeps >= max_iter:
                 break
+            nt = int(todo.sum())
+            if nt == 0:
+                break
+            if nt <= Bw // 2 and Bw > 64:
+                res[idx] = w.fn
+                if xw is not x:
+                    x[:, idx] = xw
+                idx = idx[todo]
+                xw = x[:, idx].contiguous()
+                prm = tuple(v[..., idx].contiguous() for v in (n0, Tg, E, ns, ne))
+                tolw = tol[idx].contiguous()
+                w = _NKWork(n, nt, m, dev)
+                x_try = torch.empty_like(xw)
+                bt, negam, keep = evaluate(xw, w, prm)
+                Bw = nt
+                todo = (w.fn > tolw) & ~failed[idx] & ~floor[idx]
             sweeps += 1
-            iters += todo.to(torch.int32)
+            iters[idx] += todo.to(i32)
             # ---- GMRES on J d = -F, right-preconditioned by the frozen-potential factorisation
-            w.active.copy_(todo.to(torch.int32))
+            w.active.copy_(todo.to(i32))
             w.kdim.zero_()
             w.g.zero_()
             w.g[:, 0] = w.fn
-            torch.maximum(forcing * w.fn, 0.25 * tol, out=w.target)
+            torch.maximum(forcing * w.fn, 0.25 * tolw, out=w.target)
             torch.div(w.F, -w.fn.clamp_min(1e-300)[:, None], out=w.V[0])
             for i in range(m):
                 _lib.call("bz_ee_newton_precond", grid, bt, w.active, w.V[i], w.z, w.zT)
-                _lib.call("bz_ee_potentials_cols", self.H, self.G, self.ns, self.npr, w.zT, B, w.hz, w.gz)
-                _lib.call("bz_ee_jacobian_t2", self.ns, self.npr, B, self.ccA, self.ccB, x, w.hz, w.gz, negam, w.t2)
+                _lib.call("bz_ee_potentials_cols", self.H, self.G, self.ns, self.npr, w.zT, Bw, w.hz, w.gz)
+                _lib.call("bz_ee_jacobian_t2", self.ns, self.npr, Bw, self.ccA, self.ccB, xw, w.hz, w.gz, negam,
+                          w.t2)
                 _lib.call("bz_ee_newton_jv", grid, bt, w.active, w.z, w.t2, w.Jz)
-                _lib.call("bz_arnoldi_batched", n, B, m, i, w.V, w.Jz, w.R, w.cs, w.sn, w.g, w.target, w.active,
+                _lib.call("bz_arnoldi_batched", n, Bw, m, i, w.V, w.Jz, w.R, w.cs, w.sn, w.g, w.target, w.active,
                           w.kdim, w.resid)
                 if not bool(w.active.any()):
                     break
-            krylov += w.kdim
-            _lib.call("bz_gmres_combine_batched", n, B, m, w.V, w.R, w.g, w.kdim, w.Jz)
-            w.active.copy_((w.kdim > 0).to(torch.int32))
+            krylov[idx] += w.kdim
+            _lib.call("bz_gmres_combine_batched", n, Bw, m, w.V, w.R, w.g, w.kdim, w.Jz)
+            w.active.copy_((w.kdim > 0).to(i32))
             w.z.zero_()
             w.zT.zero_()
             _lib.call("bz_ee_newton_precond", grid, bt, w.active, w.Jz, w.z, w.zT)
             # ---- Armijo backtracking on 0.5 |F|^2 per instance (bte_0d3v_batched.py:936-952)
             fn0 = w.fn.clone()
-            alpha = torch.ones(B, dtype=f64, device=dev)
+            r0w = r0[idx]
+            alpha = torch.ones(Bw, dtype=f64, device=dev)
             pending = todo.clone()
+            stopped = torch.zeros(Bw, dtype=torch.bool, device=dev)
             accepted_first = True
-            while True:
-                torch.addcmul(x, w.zT, alpha[None, :], out=x_try)
-                bt, negam, keep = evaluate(x_try)
-                ok = w.fn * w.fn <= fn0 * fn0 * (1.0 - 2.0 * line_search_c * alpha)
-                take = pending & ok
-                x[:, take] = x_try[:, take]
-                pending &= ~ok
-                if not bool(pending.any()):
-                    break
+            # first trial (alpha = 1) over the whole working set: when every instance takes it,
+            # its profiles and residuals are those of the next sweep
+            torch.addcmul(xw, w.zT, alpha[None, :], out=x_try)
+            bt, negam, keep = evaluate(x_try, w, prm)
+            ok = w.fn * w.fn <= fn0 * fn0 * (1.0 - 2.0 * line_search_c * alpha)
+            xw[:, pending & ok] = x_try[:, pending & ok]
+            pending &= ~ok
+            while bool(pending.any()):
                 accepted_first = False
                 alpha = torch.where(pending, alpha * alpha_rho, alpha)
-                flat = pending & (alpha < floor_alpha) & (fn0 <= floor_rel * r0)
-                floor |= flat
+                flat = pending & (alpha < floor_alpha) & (fn0 <= floor_rel * r0w)
                 dead = pending & (alpha < alpha_min)
-                failed |= dead
+                floor[idx[flat]] = True
+                failed[idx[dead]] = True
+                stopped |= flat | dead
                 pending &= ~(dead | flat)
-                if not bool(pending.any()):
+                pid = pending.nonzero().squeeze(1)
+                if pid.numel() == 0:
                     break
+                # later trials only over the instances still backtracking
+                xs = torch.addcmul(xw[:, pid], w.zT[:, pid], alpha[pid][None, :]).contiguous()
+                fs = self.residual_norm(xs, tuple(v[..., pid].contiguous() for v in prm), tg_scale)
+                oks = fs * fs <= fn0[pid] * fn0[pid] * (1.0 - 2.0 * line_search_c * alpha[pid])
+                tk = pid[oks]
+                xw[:, tk] = xs[:, oks]
+                pending[tk] = False
             if callback is not None:
-                callback(sweeps, w.fn.clone(), alpha, w.kdim.clone())
-            if not accepted_first or bool((failed | floor).any()):
-                bt, negam, keep = evaluate(x)
+                callback(sweeps, idx, w.fn.clone(), alpha, w.kdim.clone())
+            if not accepted_first or bool(stopped.any()):
+                bt, negam, keep = evaluate(xw, w, prm)
             else:
-                x.copy_(x_try)
-                bt, _ = s.batch_struct(x, n0, Tg, E, ns, tg_scale=tg_scale)
-        res = w.fn.clone()
-        status = torch.where(res <= tol, 0, torch.where(failed, 2, torch.where(floor, 3, 1))).to(torch.int32)
+                xw.copy_(x_try)
+                bt, _ = s.batch_struct(xw, *prm[:4], tg_scale=tg_scale)
+        res[idx] = w.fn
+        if xw is not x:
+            x[:, idx] = xw
+        status = torch.where(res <= tol, 0, torch.where(failed, 2, torch.where(floor, 3, 1))).to(i32)
         return {"iters": iters, "krylov": krylov, "status": status, "res_abs": res, "res0": r0, "outer": sweeps}
 
 

@@ -69,14 +69,16 @@ class Batched0D:
         if B not in self._scratch:
             t = self.tables
             dev = self.g.dev
-            self._scratch = {B: dict(
+            if len(self._scratch) >= 3:                 # keep the three most recent batch sizes
+                self._scratch.pop(next(iter(self._scratch)))
+            self._scratch[B] = dict(
                 kprof=torch.empty((B, t.P), dtype=torch.float64, device=dev),
                 urows=torch.empty((B, t.n, t.WP), dtype=torch.float64, device=dev),
                 vec=torch.empty((B, 4, t.n), dtype=torch.float64, device=dev),
                 iters=torch.empty(B, dtype=torch.int32, device=dev),
                 status=torch.empty(B, dtype=torch.int32, device=dev),
                 res_abs=torch.empty(B, dtype=torch.float64, device=dev),
-                res0=torch.empty(B, dtype=torch.float64, device=dev))}
+                res0=torch.empty(B, dtype=torch.float64, device=dev))
         return self._scratch[B]
 
     def batch_struct(self, x, n0, Tg, E, ns_by_n0, atol=0.0, rtol=0.0, max_iter=0, alpha_min=1e-16, alpha_rho=0.5,
