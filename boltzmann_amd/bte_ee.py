@@ -196,7 +196,7 @@ class EEOperator:
         xw, prm = x, (n0, Tg, E, ns, ne)
         w = self._ws.get((B, m))
         if w is None:
-            self._ws = {(B, m): _NKWork(n, B, m, dev)}
+            self._ws = {(B, m): _NKWork(n, B, m, dev, s.tables.P)}
             w = self._ws[(B, m)]
 
         def evaluate(xe, w, prm):
@@ -231,7 +231,7 @@ class EEOperator:
                 xw = x[:, idx].contiguous()
                 prm = tuple(v[..., idx].contiguous() for v in (n0, Tg, E, ns, ne))
                 tolw = tol[idx].contiguous()
-                w = _NKWork(n, nt, m, dev)
+                w = _NKWork(n, nt, m, dev, s.tables.P)
                 x_try = torch.empty_like(xw)
                 bt, negam, keep = evaluate(xw, w, prm)
                 Bw = nt
@@ -240,13 +240,14 @@ class EEOperator:
             iters[idx] += todo.to(i32)
             # ---- GMRES on J d = -F, right-preconditioned by the frozen-potential factorisation
             w.active.copy_(todo.to(i32))
+            _lib.call("bz_ee_newton_factor", grid, bt, w.active, w.lprof, w.z1, w.s1)
             w.kdim.zero_()
             w.g.zero_()
             w.g[:, 0] = w.fn
             torch.maximum(forcing * w.fn, 0.25 * tolw, out=w.target)
             torch.div(w.F, -w.fn.clamp_min(1e-300)[:, None], out=w.V[0])
             for i in range(m):
-                _lib.call("bz_ee_newton_precond", grid, bt, w.active, w.V[i], w.z, w.zT)
+                _lib.call("bz_ee_newton_precond_apply", grid, bt, w.active, w.lprof, w.z1, w.s1, w.V[i], w.z, w.zT)
                 _lib.call("bz_ee_potentials_cols", self.H, self.G, self.ns, self.npr, w.zT, Bw, w.hz, w.gz)
                 _lib.call("bz_ee_jacobian_t2", self.ns, self.npr, Bw, self.ccA, self.ccB, xw, w.hz, w.gz, negam,
                           w.t2)
@@ -260,7 +261,7 @@ class EEOperator:
             w.active.copy_((w.kdim > 0).to(i32))
             w.z.zero_()
             w.zT.zero_()
-            _lib.call("bz_ee_newton_precond", grid, bt, w.active, w.Jz, w.z, w.zT)
+            _lib.call("bz_ee_newton_precond_apply", grid, bt, w.active, w.lprof, w.z1, w.s1, w.Jz, w.z, w.zT)
             # ---- Armijo backtracking on 0.5 |F|^2 per instance (bte_0d3v_batched.py:936-952)
             fn0 = w.fn.clone()
             r0w = r0[idx]
@@ -311,8 +312,11 @@ class EEOperator:
 class _NKWork:
     """Device work arrays of the batched Newton-Krylov solve (B instances of size n, restart m)."""
 
-    def __init__(self, n, B, m, dev):
+    def __init__(self, n, B, m, dev, P=1):
         f = dict(dtype=torch.float64, device=dev)
+        self.lprof = torch.empty((B, P), **f)        # L of the preconditioner, profile positions
+        self.z1 = torch.empty((B, n), **f)
+        self.s1 = torch.empty(B, **f)
         self.mu = torch.empty(B, **f)
         self.F = torch.empty((B, n), **f)
         self.fn = torch.empty(B, **f)

@@ -209,15 +209,16 @@ struct FactorState {
   double w[R][W], nk[R][W], b1[R], b2[R];
 };
 
-// One W-step chunk of the factorisation running levels 0..RA-1 only.
-template <int W, int R, int RA>
+// One W-step chunk of the factorisation running levels 0..RA-1 only.  SL: also store the
+// multipliers l_rj of L at the profile positions of (r, j) in lout.
+template <int W, int R, int RA, bool SL>
 __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
                                              const double* __restrict__ kprof,
                                              const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
                                              const short* __restrict__ s_next, const double* __restrict__ muhat,
                                              double* __restrict__ Y1, double* __restrict__ Y2,
                                              double* __restrict__ s_piv, double* __restrict__ urows, int WP,
-                                             int lane) {
+                                             int lane, double* __restrict__ lout) {
 #pragma unroll
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
@@ -260,6 +261,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const double m = s.w[r][t] * rinv;
+      if (SL && s.row[r] > j && j >= s.fc[r]) lout[s.off[r] + j] = m;
 #pragma unroll
       for (int k = 1; k < W; ++k) s.w[r][(t + k) % W] = fma(-m, pv[k], s.w[r][(t + k) % W]);
       s.b1[r] = fma(-m, pb1, s.b1[r]);
@@ -307,12 +309,12 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 // values at pivot step j, which is never before the entry of row j.  U rows (1/U_jj
 // first) go to urows.  The column prefetch is branch-free so that every step issues
 // a fixed number of loads and the memory-counter waits can be counted statically.
-template <int W, int R>
+template <int W, int R, bool SL = false>
 __device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof,
                                const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
                                const short* __restrict__ s_next, const short* __restrict__ s_lev,
                                double* __restrict__ Y1, double* __restrict__ Y2, double* __restrict__ s_piv,
-                               double* __restrict__ urows, int WP, int lane) {
+                               double* __restrict__ urows, int WP, int lane, double* __restrict__ lout = nullptr) {
   const int n = g.n, bw = g.bw;
   FactorState<W, R> s;
 #pragma unroll
@@ -346,9 +348,11 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   for (int j0 = 0, ch = 0; j0 < n; j0 += W, ++ch) {
     const int levels = __builtin_amdgcn_readfirstlane((int)s_lev[ch]);
     if (R == 1 || levels <= 1) {
-      factor_chunk<W, R, 1>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane);
+      factor_chunk<W, R, 1, SL>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane,
+                                lout);
     } else {
-      factor_chunk<W, R, R>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane);
+      factor_chunk<W, R, R, SL>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane,
+                                lout);
     }
   }
 }
@@ -818,6 +822,116 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
 }
 
+// The preconditioner with the factorisation kept: once per Newton step, L (multipliers, in
+// the profile positions of the instance, lprof) and U (urows) of K_x - mu M, and
+// z1 = (K_x - mu M)^-1 M uhat with s1 = uhat.z1 (permuted order); then every application
+// is a mass product, one profile forward solve and one back substitution.
+template <int W, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_ee_factor(
+    GridDev g, BatchDev b, const int* __restrict__ active, double* __restrict__ lprof, double* __restrict__ z1,
+    double* __restrict__ s1) {
+  extern __shared__ double lds[];
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  constexpr int WP = (W + 1) & ~1;
+  constexpr int PIVN = (W + 3) & ~1;
+  const int nchunk = (n + W - 1) / W;
+  double* Z1 = lds + n;
+  double* Z2 = Z1 + n;
+  double* AV = Z2 + n + zpad(n, g.npr);
+  double* STG = AV + n;
+  double* PIV = STG + 2 * W * WP;
+  int* PO = reinterpret_cast<int*>(PIV + PIVN);
+  short* FC = reinterpret_cast<short*>(PO + (n + 1));
+  short* NX = FC + n;
+  short* LEV = NX + n;
+  const double* kprof = b.kprof + (size_t)inst * g.P;
+  double* urows = b.urows + (size_t)inst * n * WP;
+  for (int i = lane; i < n; i += 64) {
+    FC[i] = (short)g.first_col[i];
+    NX[i] = (short)g.sched_next[i];
+    Z1[i] = g.muhat[i];
+    Z2[i] = 0.0;
+  }
+  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
+  for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
+  __syncthreads();
+  factor_forward<W, R, true>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane, lprof + (size_t)inst * g.P);
+  __syncthreads();
+  back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
+  __syncthreads();
+  const double su = dot(g.uhat, Z1, n, lane);
+  for (int i = lane; i < n; i += 64) z1[(size_t)inst * n + i] = Z1[i];
+  if (lane == 0) s1[inst] = su;
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, const int* __restrict__ active,
+                                                         const double* __restrict__ lprof,
+                                                         const double* __restrict__ z1, const double* __restrict__ s1,
+                                                         const double* __restrict__ v, double* __restrict__ out,
+                                                         double* __restrict__ outT) {
+  extern __shared__ double lds[];
+  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  constexpr int WP = (W + 1) & ~1;
+  double* VV = lds;
+  double* Y1 = VV + n;
+  double* Y2 = Y1 + n;
+  double* STG = Y2 + n;
+  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
+  const double* __restrict__ L = lprof + (size_t)inst * g.P;
+  const double* urows = b.urows + (size_t)inst * n * WP;
+  for (int i = lane; i < n; i += 64) {
+    VV[i] = v[(size_t)inst * n + orig_index(i, g.ns, g.npr)];
+    Y1[i] = 0.0;
+  }
+  __syncthreads();
+  mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, VV, Y2, lane);
+  __syncthreads();
+  // forward solve with the unit lower factor, row by row (row i reads y_j, j < i)
+  for (int i = 0; i < n; ++i) {
+    const int fc = g.first_col[i], off = g.prof_off[i] - fc;
+    double sacc = 0.0;
+    for (int j = fc + lane; j < i; j += 64) sacc = fma(L[off + j], Y2[j], sacc);
+    sacc = wave_sum(sacc);
+    if (lane == 0) Y2[i] -= sacc;
+    __syncthreads();
+  }
+  back_substitute<W>(n, urows, WP, Y1, Y2, STG, lane);
+  __syncthreads();
+  const double c = -dot(g.uhat, Y2, n, lane) / s1[inst];
+  const double* Z1 = z1 + (size_t)inst * n;
+  for (int i = lane; i < n; i += 64) {
+    const int o = orig_index(i, g.ns, g.npr);
+    const double z = fma(c, Z1[i], Y2[i]);
+    out[(size_t)inst * n + o] = z;
+    if (outT) outT[(size_t)o * b.B + inst] = z;
+  }
+}
+
+template <int W, int R>
+int launch_ee_factor(const GridDev& g, const BatchDev& b, const int* active, double* lprof, double* z1, double* s1,
+                     hipStream_t stream) {
+  const size_t lds = newton_lds_bytes<W>(g.n, g.npr);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL((k_ee_factor<W, R>), dim3(b.B), dim3(64), lds, stream, g, b, active, lprof, z1, s1);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+template <int W>
+int launch_ee_apply(const GridDev& g, const BatchDev& b, const int* active, const double* lprof, const double* z1,
+                    const double* s1, const double* v, double* out, double* outT, hipStream_t stream) {
+  constexpr int WP = (W + 1) & ~1;
+  const size_t lds = (size_t)(3 * g.n + 2 * W * WP) * sizeof(double);
+  if (lds > 64 * 1024) return BZ_EINVAL;
+  hipLaunchKernelGGL((k_ee_precond_apply<W>), dim3(b.B), dim3(64), lds, stream, g, b, active, lprof, z1, s1, v, out,
+                     outT);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
 template <int W, int R>
 int launch_ee_precond(const GridDev& g, const BatchDev& b, const int* active, const double* v, double* out,
                       double* outT, hipStream_t stream) {
@@ -1248,6 +1362,45 @@ int bz_ee_newton_precond(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const in
     if (R == 3) return launch_ee_precond<17, 3>(g, b, active, v, z, zT, stream);
     if (R == 4) return launch_ee_precond<17, 4>(g, b, active, v, z, zT, stream);
   }
+  return BZ_EINVAL;
+}
+
+int bz_ee_newton_factor(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, double* lprof, double* z1,
+                        double* s1, hipStream_t stream) {
+  if (!gp || !bp || !lprof || !z1 || !s1) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  const int Wneed = g.bw + 1;
+  const int R = g.nslots / 64;
+  if (Wneed <= 8) {
+    if (R == 1) return launch_ee_factor<8, 1>(g, b, active, lprof, z1, s1, stream);
+    if (R == 2) return launch_ee_factor<8, 2>(g, b, active, lprof, z1, s1, stream);
+    if (R == 3) return launch_ee_factor<8, 3>(g, b, active, lprof, z1, s1, stream);
+  } else if (Wneed <= 11) {
+    if (R == 1) return launch_ee_factor<11, 1>(g, b, active, lprof, z1, s1, stream);
+    if (R == 2) return launch_ee_factor<11, 2>(g, b, active, lprof, z1, s1, stream);
+    if (R == 3) return launch_ee_factor<11, 3>(g, b, active, lprof, z1, s1, stream);
+  } else if (Wneed <= 17) {
+    if (R == 1) return launch_ee_factor<17, 1>(g, b, active, lprof, z1, s1, stream);
+    if (R == 2) return launch_ee_factor<17, 2>(g, b, active, lprof, z1, s1, stream);
+    if (R == 3) return launch_ee_factor<17, 3>(g, b, active, lprof, z1, s1, stream);
+    if (R == 4) return launch_ee_factor<17, 4>(g, b, active, lprof, z1, s1, stream);
+  }
+  return BZ_EINVAL;
+}
+
+int bz_ee_newton_precond_apply(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, const double* lprof,
+                               const double* z1, const double* s1, const double* v, double* z, double* zT,
+                               hipStream_t stream) {
+  if (!gp || !bp || !lprof || !z1 || !s1 || !v || !z) return BZ_EINVAL;
+  GridDev g;
+  BatchDev b;
+  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
+  const int Wneed = g.bw + 1;
+  if (Wneed <= 8) return launch_ee_apply<8>(g, b, active, lprof, z1, s1, v, z, zT, stream);
+  if (Wneed <= 11) return launch_ee_apply<11>(g, b, active, lprof, z1, s1, v, z, zT, stream);
+  if (Wneed <= 17) return launch_ee_apply<17>(g, b, active, lprof, z1, s1, v, z, zT, stream);
   return BZ_EINVAL;
 }
 

@@ -176,6 +176,17 @@ int bz_ee_newton_residual(const bz_grid0d_t* grid, const bz_batch0d_t* batch, co
 int bz_ee_newton_precond(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, const double* v,
                          double* z, double* zT, hipStream_t stream);
 
+/* The same preconditioner with the factorisation kept.  bz_ee_newton_factor (once per Newton
+ * step, after bz_ee_newton_residual): L of K_x - mu M into lprof ([B][P], the profile
+ * positions of its entries), U into batch->urows, z1 ([B][n]) = (K_x - mu M)^-1 M uhat and
+ * s1[b] = uhat.z1.  bz_ee_newton_precond_apply: z, zT as bz_ee_newton_precond from these
+ * factors (mass product, profile forward solve, back substitution). */
+int bz_ee_newton_factor(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, double* lprof,
+                        double* z1, double* s1, hipStream_t stream);
+int bz_ee_newton_precond_apply(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active,
+                               const double* lprof, const double* z1, const double* s1, const double* v, double* z,
+                               double* zT, hipStream_t stream);
+
 /* out (B, n) = J z with t2 (B, n) = T2 z from bz_ee_jacobian_t2. */
 int bz_ee_newton_jv(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, const double* z,
                     const double* t2, double* out, hipStream_t stream);
