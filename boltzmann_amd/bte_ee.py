@@ -250,7 +250,7 @@ class EEOperator:
                 _lib.call("bz_ee_newton_precond_apply", grid, bt, w.active, w.lprof, w.z1, w.s1, w.V[i], w.z, w.zT)
                 _lib.call("bz_ee_potentials_cols", self.H, self.G, self.ns, self.npr, w.zT, Bw, w.hz, w.gz)
                 _lib.call("bz_ee_jacobian_t2", self.ns, self.npr, Bw, self.ccA, self.ccB, xw, w.hz, w.gz, negam,
-                          w.t2)
+                          w.active, w.t2)
                 _lib.call("bz_ee_newton_jv", grid, bt, w.active, w.z, w.t2, w.Jz)
                 _lib.call("bz_arnoldi_batched", n, Bw, m, i, w.V, w.Jz, w.R, w.cs, w.sn, w.g, w.target, w.active,
                           w.kdim, w.resid)

@@ -249,11 +249,12 @@ __global__ void k_ee_potentials_dd(const double* __restrict__ H, const double* _
 // x, hv, gv (n, B); t2 (B, n).  cc banded: |k - p|, |r - p| <= 3.
 __global__ void k_ee_t2(int ns, int npr, int B, const double* __restrict__ ccA, const double* __restrict__ ccB,
                         const double* __restrict__ x, const double* __restrict__ hv, const double* __restrict__ gv,
-                        const double* __restrict__ negam, double* __restrict__ t2) {
+                        const double* __restrict__ negam, const int* __restrict__ active, double* __restrict__ t2) {
   const int n = ns * npr, nc = ns * ns * ns;
   const long total = (long)n * B;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int inst = (int)(e % B), row = (int)(e / B);
+    if (active && !active[inst]) continue;
     const int p = row / ns, q = row - p * ns;
     double s = 0.0;
     for (int dk = -3; dk <= 3; ++dk) {
@@ -371,10 +372,10 @@ int bz_ee_potentials_cols(const double* H, const double* G, int ns, int npr, con
 }
 
 int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* ccB, const double* x, const double* hv,
-                      const double* gv, const double* negam, double* t2, hipStream_t stream) {
+                      const double* gv, const double* negam, const int* active, double* t2, hipStream_t stream) {
   if (ns < 1 || npr < 1 || B < 1 || !ccA || !ccB || !x || !hv || !gv || !negam || !t2) return BZ_EINVAL;
   hipLaunchKernelGGL(k_ee_t2, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, ns, npr, B, ccA, ccB, x, hv,
-                     gv, negam, t2);
+                     gv, negam, active, t2);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

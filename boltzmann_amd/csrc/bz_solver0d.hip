@@ -865,6 +865,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   if (lane == 0) s1[inst] = su;
 }
 
+// Forward solve with the stored unit lower factor, row by row; the profile rows are staged
+// through LDS in chunks of whole rows (at most LBUF entries) with the next chunk's loads in
+// flight (registers) while the current chunk is swept, so the sweep waits on LDS only.
+constexpr int LBUF = 1024;
+
 template <int W>
 __global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, const int* __restrict__ active,
                                                          const double* __restrict__ lprof,
@@ -875,28 +880,72 @@ __global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, 
   const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
   if (active && !active[inst]) return;
   constexpr int WP = (W + 1) & ~1;
+  constexpr int PER = LBUF / 64;
   double* VV = lds;
   double* Y1 = VV + n;
   double* Y2 = Y1 + n;
   double* STG = Y2 + n;
+  double* SB = STG + 2 * W * WP;                 // two stages of LBUF
+  int* PO = reinterpret_cast<int*>(SB + 2 * LBUF);
+  short* FC = reinterpret_cast<short*>(PO + (n + 1));
   const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
   const double* __restrict__ L = lprof + (size_t)inst * g.P;
   const double* urows = b.urows + (size_t)inst * n * WP;
   for (int i = lane; i < n; i += 64) {
     VV[i] = v[(size_t)inst * n + orig_index(i, g.ns, g.npr)];
     Y1[i] = 0.0;
+    FC[i] = (short)g.first_col[i];
   }
+  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
   __syncthreads();
   mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, VV, Y2, lane);
+  // rows [r0, r_end(r0)) fit one stage (a row has at most n <= LBUF entries: checked at launch)
+  auto r_end = [&](int r0) {
+    int r1 = r0 + 1;
+    while (r1 < n && PO[r1 + 1] - PO[r0] <= LBUF) ++r1;
+    return r1;
+  };
+  double reg[PER];
+  int r0 = 0, r1 = r_end(0), st = 0;
+  {
+    const int e0 = PO[0], cnt = PO[r1] - e0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int o = lane + 64 * k;
+      if (o < cnt) SB[o] = L[e0 + o];
+    }
+  }
   __syncthreads();
-  // forward solve with the unit lower factor, row by row (row i reads y_j, j < i)
-  for (int i = 0; i < n; ++i) {
-    const int fc = g.first_col[i], off = g.prof_off[i] - fc;
-    double sacc = 0.0;
-    for (int j = fc + lane; j < i; j += 64) sacc = fma(L[off + j], Y2[j], sacc);
-    sacc = wave_sum(sacc);
-    if (lane == 0) Y2[i] -= sacc;
+  while (r0 < n) {
+    const int r2 = (r1 < n) ? r_end(r1) : n;
+    const int e1 = PO[r1], cnt1 = PO[r2] - e1;    // cnt1 = 0 after the last stage
+    if (cnt1 > 0) {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int o = lane + 64 * k;
+        reg[k] = L[e1 + (o < cnt1 ? o : 0)];
+      }
+    }
+    const double* S = SB + st * LBUF;
+    const int base = PO[r0];
+    for (int i = r0; i < r1; ++i) {
+      const int fc = FC[i], off = PO[i] - base - fc;
+      double sacc = 0.0;
+      for (int j = fc + lane; j < i; j += 64) sacc = fma(S[off + j], Y2[j], sacc);
+      sacc = wave_sum(sacc);
+      if (lane == 0) Y2[i] -= sacc;
+      __syncthreads();
+    }
+    double* D = SB + (st ^ 1) * LBUF;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int o = lane + 64 * k;
+      if (o < cnt1) D[o] = reg[k];
+    }
     __syncthreads();
+    st ^= 1;
+    r0 = r1;
+    r1 = r2;
   }
   back_substitute<W>(n, urows, WP, Y1, Y2, STG, lane);
   __syncthreads();
@@ -924,8 +973,9 @@ template <int W>
 int launch_ee_apply(const GridDev& g, const BatchDev& b, const int* active, const double* lprof, const double* z1,
                     const double* s1, const double* v, double* out, double* outT, hipStream_t stream) {
   constexpr int WP = (W + 1) & ~1;
-  const size_t lds = (size_t)(3 * g.n + 2 * W * WP) * sizeof(double);
-  if (lds > 64 * 1024) return BZ_EINVAL;
+  const size_t lds = (size_t)(3 * g.n + 2 * W * WP + 2 * LBUF) * sizeof(double) + (size_t)(g.n + 1) * sizeof(int) +
+                     (size_t)g.n * sizeof(short);
+  if (lds > 64 * 1024 || g.n > LBUF) return BZ_EINVAL;     // a profile row has at most n entries
   hipLaunchKernelGGL((k_ee_precond_apply<W>), dim3(b.B), dim3(64), lds, stream, g, b, active, lprof, z1, s1, v, out,
                      outT);
   BZ_LAUNCH_CHECK();

@@ -368,9 +368,9 @@ int bz_ee_potentials_cols(const double* H, const double* G, int ns, int npr, con
 /* Potential-derivative term of the e-e Jacobian (the cc_12 x product of
  * bte_0d3v_batched.py:734-743) in M-space: t2[b, (p,q)] = negam[b] sum cc_a[(p,q),(k,l),(r,s)]
  * hv[(r,s), b] x[(k,l), b] + (cc_b, gv); x, hv, gv (n, B) (hv, gv the potentials of the
- * direction), t2 (B, n). */
+ * direction), t2 (B, n); instances with active[b] == 0 are skipped (active optional). */
 int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* ccB, const double* x, const double* hv,
-                      const double* gv, const double* negam, double* t2, hipStream_t stream);
+                      const double* gv, const double* negam, const int* active, double* t2, hipStream_t stream);
 
 /* negam[b] = ne[b] gamma_a(x_b): m0 = c_m0 mass.x, kT = |c_kt top.x / m0|,
  * Lambda = c_lam kT^1.5 / sqrt(ne), gamma_a = ln(Lambda) c_gam (bte_0d3v_batched.py:652-661). */

@@ -160,7 +160,7 @@ def test_newton_krylov_pieces(setup):
     w.z.copy_(_d(v.T))
     w.zT.copy_(_d(v))
     _lib.call("bz_ee_potentials_cols", eop.H, eop.G, eop.ns, eop.npr, w.zT, B, w.hz, w.gz)
-    _lib.call("bz_ee_jacobian_t2", eop.ns, eop.npr, B, eop.ccA, eop.ccB, x, w.hz, w.gz, negam, w.t2)
+    _lib.call("bz_ee_jacobian_t2", eop.ns, eop.npr, B, eop.ccA, eop.ccB, x, w.hz, w.gz, negam, None, w.t2)
     _lib.call("bz_ee_newton_jv", grid, bt, None, w.z, w.t2, w.Jz)
     jv = Q.T @ w.Jz.cpu().numpy().T
     jv_ref = np.einsum("xab,bx->ax", J_ref, y)
