@@ -311,7 +311,7 @@ def bench_glow(args):
     backward-Euler v-step, one x-advection half step."""
     import torch
     from boltzmann_amd.glow1d import Ordinates, VStep, XAdvection
-    g, s, dev = _grid_and_solver()
+    g, s, dev = _grid_and_solver(q=11)               # spline_qpts = 11 for the glow grid
     Minv = torch.linalg.inv(s.M)
     C_en, C_gT, A_v = Minv @ sum(s.C), Minv @ s.Cg, Minv @ s.A
     tau, np0, Tg, dt = 1.0 / 13.56e6, 8e16, 300.0, 5e-5
@@ -341,7 +341,8 @@ def bench_glow(args):
         "metric": "1D3V glow step (x-half, v, x-half) per second", "value": args.steps / dt_w, "unit": "steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt_w / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "1D3V step, Nr=128 lmax=2 Nvt=32 Nx=200, E=1e4 x^3 V/m, GMRES restart 30",
+        "config": {"workload": "1D3V step, Nr=128 lmax=2 Nvt=32 Nx=200 spline_qpts=11, E=1e4 x^3 V/m, "
+                               "GMRES restart 30",
                    "dof_v": g.n, "Nx": Nx, "Nvt": Nvt},
         "gmres_iterations": vs.last_iters}), flush=True)
 
