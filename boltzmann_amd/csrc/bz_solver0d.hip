@@ -772,56 +772,6 @@ __global__ __launch_bounds__(64) void k_ee_apply(GridDev g, BatchDev b, int mode
   if (lane == 0 && nrm) nrm[inst] = sqrt(s2);
 }
 
-// Right preconditioner: z in uhat-perp with P (M^-1 (K_x - mu M) z) = v, i.e.
-// (K_x - mu M) z = M (v + c uhat) with c such that uhat.z = 0 (two right-hand sides).
-// v, out (B, n) original order; outT (n, B) the same values, when given.
-template <int W, int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_ee_precond(
-    GridDev g, BatchDev b, const int* __restrict__ active, const double* __restrict__ v, double* __restrict__ out,
-    double* __restrict__ outT) {
-  extern __shared__ double lds[];
-  const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
-  if (active && !active[inst]) return;
-  constexpr int WP = (W + 1) & ~1;
-  constexpr int PIVN = (W + 3) & ~1;
-  const int nchunk = (n + W - 1) / W;
-  double* VV = lds;
-  double* Z1 = VV + n;
-  double* Z2 = Z1 + n;
-  double* AV = Z2 + n + zpad(n, g.npr);
-  double* STG = AV + n;
-  double* PIV = STG + 2 * W * WP;
-  int* PO = reinterpret_cast<int*>(PIV + PIVN);
-  short* FC = reinterpret_cast<short*>(PO + (n + 1));
-  short* NX = FC + n;
-  short* LEV = NX + n;
-  const double* kprof = b.kprof + (size_t)inst * g.P;
-  const double* __restrict__ mprof = g.opprof + (size_t)(g.nop - 1) * g.P;
-  double* urows = b.urows + (size_t)inst * n * WP;
-  for (int i = lane; i < n; i += 64) {
-    FC[i] = (short)g.first_col[i];
-    NX[i] = (short)g.sched_next[i];
-    VV[i] = v[(size_t)inst * n + orig_index(i, g.ns, g.npr)];
-    Z1[i] = g.muhat[i];
-  }
-  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
-  for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
-  __syncthreads();
-  mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, VV, Z2, lane);
-  __syncthreads();
-  factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
-  __syncthreads();
-  back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
-  __syncthreads();
-  const double c = -dot(g.uhat, Z2, n, lane) / dot(g.uhat, Z1, n, lane);
-  for (int i = lane; i < n; i += 64) {
-    const int o = orig_index(i, g.ns, g.npr);
-    const double z = fma(c, Z1[i], Z2[i]);
-    out[(size_t)inst * n + o] = z;
-    if (outT) outT[(size_t)o * b.B + inst] = z;
-  }
-}
-
 // The preconditioner with the factorisation kept: once per Newton step, L (multipliers, in
 // the profile positions of the instance, lprof) and U (urows) of K_x - mu M, and
 // z1 = (K_x - mu M)^-1 M uhat with s1 = uhat.z1 (permuted order); then every application
@@ -978,16 +928,6 @@ int launch_ee_apply(const GridDev& g, const BatchDev& b, const int* active, cons
   if (lds > 64 * 1024 || g.n > LBUF) return BZ_EINVAL;     // a profile row has at most n entries
   hipLaunchKernelGGL((k_ee_precond_apply<W>), dim3(b.B), dim3(64), lds, stream, g, b, active, lprof, z1, s1, v, out,
                      outT);
-  BZ_LAUNCH_CHECK();
-  return BZ_OK;
-}
-
-template <int W, int R>
-int launch_ee_precond(const GridDev& g, const BatchDev& b, const int* active, const double* v, double* out,
-                      double* outT, hipStream_t stream) {
-  const size_t lds = newton_lds_bytes<W>(g.n, g.npr);
-  if (lds > 64 * 1024) return BZ_EINVAL;
-  hipLaunchKernelGGL((k_ee_precond<W, R>), dim3(b.B), dim3(64), lds, stream, g, b, active, v, out, outT);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }
@@ -1264,6 +1204,21 @@ int to_dev(const bz_grid0d_t* gp, const bz_batch0d_t* bp, GridDev& g, BatchDev& 
   return BZ_OK;
 }
 
+// Window widths W (>= bw + 1) and register slot levels R (= nslots / 64) the solver kernels
+// are instantiated for; X(W, R) is expanded for each pair of the matching window.
+#define BZ_FOR_WINDOW(Wneed, X)                                      \
+  if ((Wneed) <= 8) {                                                \
+    X(8, 1) X(8, 2) X(8, 3)                                          \
+  } else if ((Wneed) <= 11) {                                        \
+    X(11, 1) X(11, 2) X(11, 3)                                       \
+  } else if ((Wneed) <= 12) {                                        \
+    X(12, 1) X(12, 2) X(12, 3)                                       \
+  } else if ((Wneed) <= 17) {                                        \
+    X(17, 1) X(17, 2) X(17, 3) X(17, 4)                              \
+  } else if ((Wneed) <= 20) {                                        \
+    X(20, 2) X(20, 3) X(20, 4)                                       \
+  }
+
 }  // namespace
 
 extern "C" {
@@ -1308,19 +1263,10 @@ int bz_transient_step_0d(const bz_grid0d_t* gp, const bz_batch0d_t* bp, double d
   if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
   const int Wneed = g.bw + 1;
   const int R = g.nslots / 64;
-  if (Wneed <= 8) {
-    if (R == 1) return launch_transient<8, 1>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 2) return launch_transient<8, 2>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 3) return launch_transient<8, 3>(g, b, dt_tau, ustart, avg, stream);
-  } else if (Wneed <= 11) {
-    if (R == 1) return launch_transient<11, 1>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 2) return launch_transient<11, 2>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 3) return launch_transient<11, 3>(g, b, dt_tau, ustart, avg, stream);
-  } else if (Wneed <= 17) {
-    if (R == 2) return launch_transient<17, 2>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 3) return launch_transient<17, 3>(g, b, dt_tau, ustart, avg, stream);
-    if (R == 4) return launch_transient<17, 4>(g, b, dt_tau, ustart, avg, stream);
-  }
+#define BZ_X(W_, R_) \
+  if (R == R_) return launch_transient<W_, R_>(g, b, dt_tau, ustart, avg, stream);
+  BZ_FOR_WINDOW(Wneed, BZ_X)
+#undef BZ_X
   return BZ_EINVAL;
 }
 
@@ -1329,20 +1275,10 @@ static int steady_dispatch(const GridDev& g, const BatchDev& b, hipStream_t stre
   if (!(b.alpha_rho > 0.0 && b.alpha_rho < 1.0) || !(b.alpha_min > 0.0)) return BZ_EINVAL;
   const int Wneed = g.bw + 1;
   const int R = g.nslots / 64;
-  if (Wneed <= 8) {
-    if (R == 1) return launch_newton<8, 1>(g, b, stream, ee);
-    if (R == 2) return launch_newton<8, 2>(g, b, stream, ee);
-    if (R == 3) return launch_newton<8, 3>(g, b, stream, ee);
-  } else if (Wneed <= 11) {
-    if (R == 1) return launch_newton<11, 1>(g, b, stream, ee);
-    if (R == 2) return launch_newton<11, 2>(g, b, stream, ee);
-    if (R == 3) return launch_newton<11, 3>(g, b, stream, ee);
-  } else if (Wneed <= 17) {
-    if (R == 1) return launch_newton<17, 1>(g, b, stream, ee);
-    if (R == 2) return launch_newton<17, 2>(g, b, stream, ee);
-    if (R == 3) return launch_newton<17, 3>(g, b, stream, ee);
-    if (R == 4) return launch_newton<17, 4>(g, b, stream, ee);
-  }
+#define BZ_X(W_, R_) \
+  if (R == R_) return launch_newton<W_, R_>(g, b, stream, ee);
+  BZ_FOR_WINDOW(Wneed, BZ_X)
+#undef BZ_X
   return BZ_EINVAL;
 }
 
@@ -1390,31 +1326,6 @@ int bz_ee_newton_residual(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const b
   return BZ_OK;
 }
 
-int bz_ee_newton_precond(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, const double* v, double* z,
-                         double* zT, hipStream_t stream) {
-  if (!gp || !bp || !v || !z) return BZ_EINVAL;
-  GridDev g;
-  BatchDev b;
-  if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
-  const int Wneed = g.bw + 1;
-  const int R = g.nslots / 64;
-  if (Wneed <= 8) {
-    if (R == 1) return launch_ee_precond<8, 1>(g, b, active, v, z, zT, stream);
-    if (R == 2) return launch_ee_precond<8, 2>(g, b, active, v, z, zT, stream);
-    if (R == 3) return launch_ee_precond<8, 3>(g, b, active, v, z, zT, stream);
-  } else if (Wneed <= 11) {
-    if (R == 1) return launch_ee_precond<11, 1>(g, b, active, v, z, zT, stream);
-    if (R == 2) return launch_ee_precond<11, 2>(g, b, active, v, z, zT, stream);
-    if (R == 3) return launch_ee_precond<11, 3>(g, b, active, v, z, zT, stream);
-  } else if (Wneed <= 17) {
-    if (R == 1) return launch_ee_precond<17, 1>(g, b, active, v, z, zT, stream);
-    if (R == 2) return launch_ee_precond<17, 2>(g, b, active, v, z, zT, stream);
-    if (R == 3) return launch_ee_precond<17, 3>(g, b, active, v, z, zT, stream);
-    if (R == 4) return launch_ee_precond<17, 4>(g, b, active, v, z, zT, stream);
-  }
-  return BZ_EINVAL;
-}
-
 int bz_ee_newton_factor(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int* active, double* lprof, double* z1,
                         double* s1, hipStream_t stream) {
   if (!gp || !bp || !lprof || !z1 || !s1) return BZ_EINVAL;
@@ -1423,20 +1334,10 @@ int bz_ee_newton_factor(const bz_grid0d_t* gp, const bz_batch0d_t* bp, const int
   if (to_dev(gp, bp, g, b) != BZ_OK) return BZ_EINVAL;
   const int Wneed = g.bw + 1;
   const int R = g.nslots / 64;
-  if (Wneed <= 8) {
-    if (R == 1) return launch_ee_factor<8, 1>(g, b, active, lprof, z1, s1, stream);
-    if (R == 2) return launch_ee_factor<8, 2>(g, b, active, lprof, z1, s1, stream);
-    if (R == 3) return launch_ee_factor<8, 3>(g, b, active, lprof, z1, s1, stream);
-  } else if (Wneed <= 11) {
-    if (R == 1) return launch_ee_factor<11, 1>(g, b, active, lprof, z1, s1, stream);
-    if (R == 2) return launch_ee_factor<11, 2>(g, b, active, lprof, z1, s1, stream);
-    if (R == 3) return launch_ee_factor<11, 3>(g, b, active, lprof, z1, s1, stream);
-  } else if (Wneed <= 17) {
-    if (R == 1) return launch_ee_factor<17, 1>(g, b, active, lprof, z1, s1, stream);
-    if (R == 2) return launch_ee_factor<17, 2>(g, b, active, lprof, z1, s1, stream);
-    if (R == 3) return launch_ee_factor<17, 3>(g, b, active, lprof, z1, s1, stream);
-    if (R == 4) return launch_ee_factor<17, 4>(g, b, active, lprof, z1, s1, stream);
-  }
+#define BZ_X(W_, R_) \
+  if (R == R_) return launch_ee_factor<W_, R_>(g, b, active, lprof, z1, s1, stream);
+  BZ_FOR_WINDOW(Wneed, BZ_X)
+#undef BZ_X
   return BZ_EINVAL;
 }
 
@@ -1450,7 +1351,9 @@ int bz_ee_newton_precond_apply(const bz_grid0d_t* gp, const bz_batch0d_t* bp, co
   const int Wneed = g.bw + 1;
   if (Wneed <= 8) return launch_ee_apply<8>(g, b, active, lprof, z1, s1, v, z, zT, stream);
   if (Wneed <= 11) return launch_ee_apply<11>(g, b, active, lprof, z1, s1, v, z, zT, stream);
+  if (Wneed <= 12) return launch_ee_apply<12>(g, b, active, lprof, z1, s1, v, z, zT, stream);
   if (Wneed <= 17) return launch_ee_apply<17>(g, b, active, lprof, z1, s1, v, z, zT, stream);
+  if (Wneed <= 20) return launch_ee_apply<20>(g, b, active, lprof, z1, s1, v, z, zT, stream);
   return BZ_EINVAL;
 }
 
@@ -1471,7 +1374,9 @@ int bz_solver0d_window(int bw) {
   const int Wneed = bw + 1;
   if (Wneed <= 8) return 8;
   if (Wneed <= 11) return 11;
+  if (Wneed <= 12) return 12;
   if (Wneed <= 17) return 17;
+  if (Wneed <= 20) return 20;
   return -1;
 }
 

@@ -171,16 +171,12 @@ int bz_ee_newton_residual(const bz_grid0d_t* grid, const bz_batch0d_t* batch, co
                           double* F, double* fnorm, hipStream_t stream);
 
 /* Right preconditioner of the Newton system: z (B, n) orthogonal to uhat with
- * P M^-1 (K_x - mu M) z = v, from the profiles left by bz_ee_newton_residual; zT (n, B),
- * optional, the same values.  Systems with active[b] == 0 are skipped (active optional). */
-int bz_ee_newton_precond(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, const double* v,
-                         double* z, double* zT, hipStream_t stream);
-
-/* The same preconditioner with the factorisation kept.  bz_ee_newton_factor (once per Newton
- * step, after bz_ee_newton_residual): L of K_x - mu M into lprof ([B][P], the profile
- * positions of its entries), U into batch->urows, z1 ([B][n]) = (K_x - mu M)^-1 M uhat and
- * s1[b] = uhat.z1.  bz_ee_newton_precond_apply: z, zT as bz_ee_newton_precond from these
- * factors (mass product, profile forward solve, back substitution). */
+ * P M^-1 (K_x - mu M) z = v, from the profiles left by bz_ee_newton_residual.
+ * bz_ee_newton_factor (once per Newton step): L of K_x - mu M into lprof ([B][P], the
+ * profile positions of its entries), U into batch->urows, z1 ([B][n]) = (K_x - mu M)^-1 M uhat
+ * and s1[b] = uhat.z1.  bz_ee_newton_precond_apply: z (B, n) and, optional, zT (n, B) with the
+ * same values, from these factors (mass product, profile forward solve, back substitution).
+ * Systems with active[b] == 0 are skipped (active optional). */
 int bz_ee_newton_factor(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active, double* lprof,
                         double* z1, double* s1, hipStream_t stream);
 int bz_ee_newton_precond_apply(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const int* active,
