@@ -27,7 +27,8 @@ class Grid0D(ctypes.Structure):
                 ("P", c_int), ("nslots", c_int),
                 ("first_col", c_void_p), ("prof_off", c_void_p), ("opprof", c_void_p), ("slot_first", c_void_p),
                 ("sched_next", c_void_p), ("muhat", c_void_p), ("uhat", c_void_p), ("u", c_void_p),
-                ("wsp", c_void_p), ("mchol", c_void_p), ("chunk_levels", c_void_p), ("ent_rc", c_void_p)]
+                ("wsp", c_void_p), ("mchol", c_void_p), ("chunk_levels", c_void_p), ("ent_rc", c_void_p),
+                ("max_row", c_int)]
 
 
 class Batch0D(ctypes.Structure):

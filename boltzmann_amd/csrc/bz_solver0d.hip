@@ -34,7 +34,7 @@ using namespace bz;
 constexpr int kMaxOps = 16;
 
 struct GridDev {
-  int n, ns, npr, bw, S, nop, P, nslots;
+  int n, ns, npr, bw, S, nop, P, nslots, max_row;
   const int* first_col;     // [n]
   const int* prof_off;      // [n+1]
   const double* opprof;     // [nop][P]
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, 
   for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
   __syncthreads();
   mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, VV, Y2, lane);
-  // rows [r0, r_end(r0)) fit one stage (a row has at most n <= LBUF entries: checked at launch)
+  // rows [r0, r_end(r0)) fit one stage (every row has at most LBUF entries: checked at launch)
   auto r_end = [&](int r0) {
     int r1 = r0 + 1;
     while (r1 < n && PO[r1 + 1] - PO[r0] <= LBUF) ++r1;
@@ -925,7 +925,7 @@ int launch_ee_apply(const GridDev& g, const BatchDev& b, const int* active, cons
   constexpr int WP = (W + 1) & ~1;
   const size_t lds = (size_t)(3 * g.n + 2 * W * WP + 2 * LBUF) * sizeof(double) + (size_t)(g.n + 1) * sizeof(int) +
                      (size_t)g.n * sizeof(short);
-  if (lds > 64 * 1024 || g.n > LBUF) return BZ_EINVAL;     // a profile row has at most n entries
+  if (lds > 64 * 1024 || g.max_row > LBUF) return BZ_EINVAL;
   hipLaunchKernelGGL((k_ee_precond_apply<W>), dim3(b.B), dim3(64), lds, stream, g, b, active, lprof, z1, s1, v, out,
                      outT);
   BZ_LAUNCH_CHECK();
@@ -1187,12 +1187,14 @@ __global__ void k_mchol_perm(const double* __restrict__ lrev, int ns, int npr, d
 int to_dev(const bz_grid0d_t* gp, const bz_batch0d_t* bp, GridDev& g, BatchDev& b) {
   g.n = gp->n; g.ns = gp->ns; g.npr = gp->npr; g.bw = gp->bw; g.S = gp->S; g.nop = gp->nop; g.P = gp->P;
   g.nslots = gp->nslots;
+  g.max_row = gp->max_row;
   g.first_col = gp->first_col; g.prof_off = gp->prof_off; g.opprof = gp->opprof; g.slot_first = gp->slot_first;
   g.sched_next = gp->sched_next; g.muhat = gp->muhat; g.uhat = gp->uhat; g.u = gp->u; g.wsp = gp->wsp;
   g.mchol = gp->mchol; g.chunk_levels = gp->chunk_levels; g.ent_rc = gp->ent_rc;
   if (g.n != g.ns * g.npr || g.n < 8 || g.n > 32767 || g.nop != g.S + 3 || g.nop > kMaxOps || g.S < 1)
     return BZ_EINVAL;
   if (g.nslots < 64 || g.nslots % 64 != 0) return BZ_EINVAL;
+  if (g.max_row < 1 || g.max_row > g.n) return BZ_EINVAL;
   b.B = bp->B; b.n0 = bp->n0; b.Tg = bp->Tg; b.E = bp->E; b.ns_by_n0 = bp->ns_by_n0; b.x = bp->x;
   b.kprof = bp->kprof; b.urows = bp->urows; b.vec = bp->vec; b.iters = bp->iters; b.status = bp->status;
   b.res_abs = bp->res_abs; b.res0 = bp->res0; b.atol = bp->atol; b.rtol = bp->rtol; b.alpha_min = bp->alpha_min;
@@ -1216,7 +1218,7 @@ int to_dev(const bz_grid0d_t* gp, const bz_batch0d_t* bp, GridDev& g, BatchDev& 
   } else if ((Wneed) <= 17) {                                        \
     X(17, 1) X(17, 2) X(17, 3) X(17, 4)                              \
   } else if ((Wneed) <= 20) {                                        \
-    X(20, 2) X(20, 3) X(20, 4)                                       \
+    X(20, 1) X(20, 2) X(20, 3) X(20, 4)                              \
   }
 
 }  // namespace

@@ -189,7 +189,8 @@ class SolverTables:
             opprof=self.opprof.data_ptr(), slot_first=self.slot_first.data_ptr(),
             sched_next=self.sched_next.data_ptr(), muhat=self.muhat.data_ptr(), uhat=self.uhat.data_ptr(),
             u=self.u.data_ptr(), wsp=self.wsp.data_ptr(), mchol=self.mchol.data_ptr(),
-            chunk_levels=self.chunk_levels.data_ptr(), ent_rc=self.ent_rc.data_ptr())
+            chunk_levels=self.chunk_levels.data_ptr(), ent_rc=self.ent_rc.data_ptr(),
+            max_row=int(np.diff(st["prof_off"]).max()))
 
     def scratch_bytes(self, B):
         return B * (self.P + self.n * self.WP + 4 * self.n) * 8

@@ -65,26 +65,30 @@ class Batched0D:
     def initial(self, B):
         return self.h0.reshape(-1, 1).repeat(1, B).contiguous()
 
+    def new_scratch(self, B):
+        """Scratch of B instances owned by the caller (for batches in flight concurrently)."""
+        t = self.tables
+        dev = self.g.dev
+        return dict(
+            kprof=torch.empty((B, t.P), dtype=torch.float64, device=dev),
+            urows=torch.empty((B, t.n, t.WP), dtype=torch.float64, device=dev),
+            vec=torch.empty((B, 4, t.n), dtype=torch.float64, device=dev),
+            iters=torch.empty(B, dtype=torch.int32, device=dev),
+            status=torch.empty(B, dtype=torch.int32, device=dev),
+            res_abs=torch.empty(B, dtype=torch.float64, device=dev),
+            res0=torch.empty(B, dtype=torch.float64, device=dev))
+
     def _get_scratch(self, B):
         if B not in self._scratch:
-            t = self.tables
-            dev = self.g.dev
-            if len(self._scratch) >= 3:                 # keep the three most recent batch sizes
+            if len(self._scratch) >= 3:                # keep the three most recent batch sizes
                 self._scratch.pop(next(iter(self._scratch)))
-            self._scratch[B] = dict(
-                kprof=torch.empty((B, t.P), dtype=torch.float64, device=dev),
-                urows=torch.empty((B, t.n, t.WP), dtype=torch.float64, device=dev),
-                vec=torch.empty((B, 4, t.n), dtype=torch.float64, device=dev),
-                iters=torch.empty(B, dtype=torch.int32, device=dev),
-                status=torch.empty(B, dtype=torch.int32, device=dev),
-                res_abs=torch.empty(B, dtype=torch.float64, device=dev),
-                res0=torch.empty(B, dtype=torch.float64, device=dev))
+            self._scratch[B] = self.new_scratch(B)
         return self._scratch[B]
 
     def batch_struct(self, x, n0, Tg, E, ns_by_n0, atol=0.0, rtol=0.0, max_iter=0, alpha_min=1e-16, alpha_rho=0.5,
-                     ls_c=1e-4, mode=1, tg_scale=1.0, tol_abs=None):
+                     ls_c=1e-4, mode=1, tg_scale=1.0, tol_abs=None, scratch=None):
         """bz_batch0d_t over x (n, B) and the per-instance parameters, with this solver's
-        scratch for B instances; returns (struct, scratch dict)."""
+        scratch for B instances (or the given one, from new_scratch); returns (struct, scratch)."""
         B = x.shape[1]
         assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
         for v in (n0, Tg, E):
@@ -92,7 +96,8 @@ class Batched0D:
         assert ns_by_n0.shape == (self.S, B) and ns_by_n0.is_contiguous()
         if tol_abs is not None:
             assert tol_abs.shape == (B,) and tol_abs.dtype == torch.float64 and tol_abs.is_contiguous()
-        sc = self._get_scratch(B)
+        sc = self._get_scratch(B) if scratch is None else scratch
+        assert sc["kprof"].shape == (B, self.tables.P)
         bt = _lib.Batch0D(B=B, n0=n0.data_ptr(), Tg=Tg.data_ptr(), E=E.data_ptr(), ns_by_n0=ns_by_n0.data_ptr(),
                           x=x.data_ptr(), kprof=sc["kprof"].data_ptr(), urows=sc["urows"].data_ptr(),
                           vec=sc["vec"].data_ptr(), iters=sc["iters"].data_ptr(), status=sc["status"].data_ptr(),
@@ -104,13 +109,13 @@ class Batched0D:
         return bt, sc
 
     def solve_raw(self, x, n0, Tg, E, ns_by_n0, atol=1e-10, rtol=1e-10, max_iter=30, alpha_min=1e-16,
-                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None, tol_abs=None, ee=None):
+                  alpha_rho=0.5, ls_c=1e-4, mode=1, tg_scale=1.0, stream=None, tol_abs=None, ee=None, scratch=None):
         """In-place Newton on x (n, B); returns the scratch dict (iters, status, res_abs, res0).
         tol_abs: optional (B,) per-instance absolute tolerance; ee: optional _lib.EE0D with the
         frozen e-e operator of every instance (bz_steady_solve_0d_ee)."""
         bt, sc = self.batch_struct(x, n0, Tg, E, ns_by_n0, atol=atol, rtol=rtol, max_iter=max_iter,
                                    alpha_min=alpha_min, alpha_rho=alpha_rho, ls_c=ls_c, mode=mode,
-                                   tg_scale=tg_scale, tol_abs=tol_abs)
+                                   tg_scale=tg_scale, tol_abs=tol_abs, scratch=scratch)
         if ee is None:
             _lib.call("bz_steady_solve_0d", self.tables.struct, bt, stream=stream)
         else:

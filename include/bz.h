@@ -96,6 +96,7 @@ typedef struct {
   const double* mchol;    /* [n][4] banded Cholesky rows of M                   */
   const int* chunk_levels;    /* [ceil(n/W)] register levels alive per W-step chunk */
   const unsigned* ent_rc;     /* [P] (row << 16) | column of every profile entry    */
+  int max_row;                /* longest row profile (entries)                      */
 } bz_grid0d_t;
 
 /* Per-batch parameters, state and scratch (device pointers). */

@@ -185,3 +185,36 @@ def test_newton_krylov_pieces(setup):
         # depends on it
         assert e <= 1e-7
     np.testing.assert_allclose(w.zT.cpu().numpy(), z, rtol=0, atol=1e-14 * np.abs(z).max())
+
+
+def test_steady_state_with_ee_lmax4():
+    """e-e at lmax 4 (window 20 of the solver, 125 harmonic triples) against the oracle."""
+    from boltzmann_amd.bte_ee import EEOperator
+    from boltzmann_amd.ee_terms_table import TERMS
+    from boltzmann_amd.solver0d import Batched0D
+    print("case: e-e steady state Nr=12 lmax=4 B=2", flush=True)
+    ops, g, procs = oracle_batched(12, 4)
+    vg = product_grid(g)
+    s = Batched0D(vg, product_xsecs(), ee_pattern=True)
+    eop = EEOperator(vg, s)
+    B = 2
+    prm = instance_params(B, seed=21)
+    prm["ne"] = prm["n0"] * np.array([1e-2, 1e-4])
+    x0, _, _ = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, 1e-13, 40, jacobian="exact")
+    hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
+    A, Bt = eeo.cc_tensors(g, TERMS)
+    ops.ee = {"cc": eeo.ee_operator(ops.Minv, A, Bt, hl, gl)}
+    try:
+        _, q_ref, _ = s0.steady_state_solve_each(ops, prm, x0, 0.0, 1e-14, 60, jacobian="exact")
+    finally:
+        ops.ee = None
+    x = _d(x0)
+    n0, Tg, E, ns, ne = (_d(prm[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
+    out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-15, 60)
+    st, ra, r0 = (out[k].cpu().numpy() for k in ("status", "res_abs", "res0"))
+    print("status", st, "|F|/|F0|", ra / r0, "steps", out["iters"].cpu().numpy(), flush=True)
+    assert np.all((st == 0) | ((st == 3) & (ra <= 1e-12 * r0)))
+    h, q = s.qois(x, n0, E)
+    errs = qoi_errors({k: v.cpu().numpy() for k, v in q.items()}, q_ref)
+    print("qoi errors", errs, flush=True)
+    assert max(errs.values()) < 1e-8, errs

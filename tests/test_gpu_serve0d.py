@@ -1,0 +1,111 @@
+"""Serving loop (boltzmann_amd/serve0d.py): continuous batching of mixed 0D requests (grid
+Nr/lmax, process count, E/N, e-e) under an HBM budget.
+
+* each result is the same alone and in the mix (bitwise: instances do not interact);
+* a served request agrees with the oracle's solve of the same instance;
+* the budget bounds the memory in flight, and a request that cannot fit is refused;
+* arrivals in real time are all served, with latencies reported.
+"""
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+
+from conftest import XSEC_FILE
+from helpers import oracle_batched, qoi_errors
+from oracle import solver0d as s0
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(16, 1, 1), (16, 2, 3), (24, 1, 3), (12, 4, 3)]
+
+
+def _requests(n=16, seed=5, ee_every=5):
+    from boltzmann_amd.serve0d import Request
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        Nr, lmax, nproc = GRIDS[i % len(GRIDS)]
+        ion = 1e-3 if (ee_every and i % ee_every == 0) else 0.0
+        out.append(Request(i, Nr, lmax, nproc, float(10.0 ** rng.uniform(0.0, 3.0)), ion,
+                           fac=tuple(float(v) for v in rng.uniform(0.8, 1.2, 3))))
+    return out
+
+
+def _same(a, b, exact):
+    if exact:
+        assert (a.energy, a.mobility, a.diffusion, a.status) == (b.energy, b.mobility, b.diffusion, b.status)
+        assert np.array_equal(a.rates, b.rates)
+    else:
+        np.testing.assert_allclose([a.energy, a.mobility, a.diffusion], [b.energy, b.mobility, b.diffusion],
+                                   rtol=1e-13)
+        np.testing.assert_allclose(a.rates, b.rates, rtol=1e-13, atol=1e-13 * np.abs(b.rates).max())
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from boltzmann_amd.serve0d import ServingEngine
+    return ServingEngine(XSEC_FILE, 8 << 30, max_iter=40)
+
+
+def test_alone_equals_mix(engine):
+    print("case: serving mix of 16 requests over 4 grids (e-e on 4 of them, one at lmax 4)", flush=True)
+    reqs = _requests()
+    mix, st = engine.run(reqs, realtime=False)
+    print("mix stats", st, flush=True)
+    print("status", [(r.rid, r.status) for r in mix], flush=True)
+    # a request without a fixed point (status 2) is reported as such, alone and in the mix
+    assert st["converged"] >= len(reqs) - 1
+    for r in reqs:
+        alone, _ = engine.run([dataclasses.replace(r, arrival=0.0)], realtime=False)
+        _same(alone[0], mix[r.rid], exact=True)
+
+
+def test_served_request_matches_oracle(engine):
+    from boltzmann_amd.serve0d import Request
+    print("case: served request vs oracle Nr=24 lmax=1, 3 processes", flush=True)
+    reqs = [Request(i, 24, 1, 3, eb, 0.0, fac=(1.0, 0.9, 1.1)) for i, eb in enumerate((3.0, 60.0, 700.0))]
+    res, _ = engine.run(reqs, realtime=False)
+    ops, g, procs = oracle_batched(24, 1)
+    prm = {"n0": np.full(3, 3.22e22), "Tg": np.full(3, 300.0), "E": np.array([r.EbyN * 3.22e22 * 1e-21 for r in reqs]),
+           "ns_by_n0": np.array([[1.0] * 3, [0.9] * 3, [1.1] * 3])}
+    _, q_ref, _ = s0.steady_state_solve_each(ops, prm, ops.initial(3), 0.0, 1e-14, 40, jacobian="exact")
+    q = {"energy": np.array([r.energy for r in res]), "mobility": np.array([r.mobility for r in res]),
+         "diffusion": np.array([r.diffusion for r in res]), "rates": np.stack([r.rates for r in res], axis=1)}
+    errs = qoi_errors(q, q_ref)
+    assert max(errs.values()) < 1e-8, errs
+
+
+def test_budget_bounds_memory_in_flight(engine):
+    from boltzmann_amd.serve0d import ServingEngine
+    print("case: serving under a budget of a few instances", flush=True)
+    reqs = _requests(12, seed=9, ee_every=0)
+    full, _ = engine.run(reqs, realtime=False)
+    per = max(engine.grid(k).bytes_per_instance(False, 30) for k in GRIDS)
+    small = ServingEngine(XSEC_FILE, int(2.5 * per), max_iter=40)
+    res, st = small.run(reqs, realtime=False)
+    print("budget stats", st, flush=True)
+    assert st["peak_bytes"] <= small.budget
+    assert st["batches"] >= math.ceil(len(reqs) / 2)
+    for a, b in zip(res, full):
+        _same(a, b, exact=True)
+    big = max(reqs, key=lambda r: engine.grid((r.Nr, r.lmax, r.nproc)).bytes_per_instance(False, 30))
+    with pytest.raises(ValueError):
+        ServingEngine(XSEC_FILE, per // 2).run([big], realtime=False)
+
+
+def test_realtime_arrivals(engine):
+    print("case: serving 24 requests arriving over 40 ms in real time", flush=True)
+    reqs = _requests(24, seed=11, ee_every=8)
+    rng = np.random.default_rng(3)
+    arr = np.sort(rng.uniform(0.0, 0.04, len(reqs)))
+    reqs = [dataclasses.replace(r, arrival=float(a)) for r, a in zip(reqs, arr)]
+    res, st = engine.run(reqs, realtime=True)
+    print("realtime stats", st, flush=True)
+    assert st["converged"] >= len(reqs) - 2
+    assert all(r.latency >= 0.0 and r.done >= r.admitted for r in res)
+    # the same instances served in the mix equal the offline results
+    off, _ = engine.run([dataclasses.replace(r, arrival=0.0) for r in reqs], realtime=False)
+    for a, b in zip(res, off):
+        _same(a, b, exact=True)
