@@ -17,6 +17,7 @@ converged instances count in solves/s: status 0, or a stop with |r| / |r0| < 1e-
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
 """
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -347,10 +348,52 @@ def bench_glow(args):
         "gmres_iterations": vs.last_iters}), flush=True)
 
 
+def bench_serve(args):
+    """Mixed 0D requests served in real time (boltzmann_amd/serve0d.py): Poisson arrivals at
+    rate --rate, each request drawing Nr in {64, 128, 256}, lmax in {1, 2, 4}, 1-3 processes,
+    E/N log-uniform over 1-1000 Td, e-e with probability 0.3 (ion_deg 10^U(-6,-2)); admission
+    by an HBM budget of 64 GiB. Reports completed solves per second and p50/p99 latency
+    (arrival to QoIs on the host)."""
+    import torch
+    from boltzmann_amd.serve0d import Request, ServingEngine
+    rng = np.random.default_rng(SEED)
+    N = args.requests
+    arr = np.cumsum(rng.exponential(1.0 / args.rate, N))
+    reqs = []
+    for i in range(N):
+        ee = rng.uniform() < 0.3
+        reqs.append(Request(i, int(rng.choice([64, 128, 256])), int(rng.choice([1, 2, 4])), int(rng.integers(1, 4)),
+                            float(10.0 ** rng.uniform(0.0, 3.0)),
+                            float(10.0 ** rng.uniform(-6.0, -2.0)) if ee else 0.0,
+                            fac=tuple(float(v) for v in rng.uniform(0.8, 1.2, 3)), arrival=float(arr[i])))
+    eng = ServingEngine(XSEC, 64 << 30, Te=TE, ev_max=EVMAX, q=Q, max_iter=40)
+    t0 = time.perf_counter()
+    for r in reqs:                                     # per-grid assembly, outside the timed run
+        g = eng.grid((r.Nr, r.lmax, r.nproc))
+        if r.ion_deg > 0:
+            g.ee()
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    warm = [dataclasses.replace(r, rid=N + j, arrival=0.0) for j, r in enumerate(reqs[:64])]
+    eng.run(warm, realtime=False)
+    res, st = eng.run(reqs, realtime=True)
+    print(json.dumps({
+        "metric": "0D EDF solves/s served (mixed grids, e-e p=0.3)", "value": st["solves_per_s"], "unit": "solves/s",
+        "n_gpus": 1, "steps": N, "warmup": len(warm), "ms_per_step": st["makespan_s"] / N * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals, Nr {64,128,256} x lmax {1,2,4} x 1-3 processes, "
+                               "E/N 1-1000 Td, e-e p=0.3, HBM budget 64 GiB", "requests": N, "rate_per_s": args.rate},
+        "p50_latency_ms": st["p50_latency_s"] * 1e3, "p99_latency_ms": st["p99_latency_s"] * 1e3,
+        "batches": st["batches"], "converged": st["converged"], "peak_bytes": st["peak_bytes"],
+        "grids": st["grids"], "setup_s": setup}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="steady", choices=["steady", "ee", "transient", "glow"],
-                    help="steady: the batched 0D steady state (default); ee, transient, glow: the other paths")
+    ap.add_argument("--workload", default="steady", choices=["steady", "ee", "transient", "glow", "serve"],
+                    help="steady: the batched 0D steady state (default); ee, transient, glow, serve: the other paths")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: mean arrival rate (requests/s)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -376,7 +419,7 @@ def main():
               f"trivially across GPUs; this build measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "steady":
-        {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow}[args.workload](args)
+        {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow, "serve": bench_serve}[args.workload](args)
         return
 
     import torch

@@ -25,6 +25,7 @@ busy meanwhile.
 Instances do not interact inside a batch (every kernel works on one instance per wavefront,
 or elementwise), so a result does not depend on which other requests share its batch.
 """
+import concurrent.futures
 import math
 import time
 from dataclasses import dataclass, field
@@ -101,7 +102,8 @@ class _Grid:
 
 class ServingEngine:
     def __init__(self, xsec_file, hbm_budget, Te=4.0, ev_max=80.0, q=5, max_batch=4096, rtol=1e-12,
-                 max_iter=30, ee_rtol=1e-15, ee_max_iter=60, restart=30, streams=4, device=None):
+                 max_iter=30, ee_rtol=1e-15, ee_max_iter=60, restart=30, streams=4, ee_min_batch=256,
+                 ee_max_wait=0.05, device=None):
         self.dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.xs = xsections_from_file(xsec_file, device=self.dev)
         self.Te, self.ev_max, self.q = Te, ev_max, q
@@ -112,6 +114,11 @@ class ServingEngine:
         self.grids = {}
         self.stats = {}
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(max(1, int(streams)))]
+        # e-e batches (host-orchestrated Newton-Krylov) run one at a time in a worker thread, so
+        # the loop keeps admitting and retiring the asynchronous batches meanwhile; they wait
+        # in their queue for ee_min_batch requests or ee_max_wait seconds
+        self._ee_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self.ee_min_batch, self.ee_max_wait = int(ee_min_batch), float(ee_max_wait)
 
     def grid(self, key):
         if key not in self.grids:
@@ -130,7 +137,7 @@ class ServingEngine:
         ne = f([r.ion_deg * r.n0 for r in reqs])
         return n0, Tg, E, ns, ne
 
-    def _launch(self, key, reqs, ee, stream):
+    def _solve(self, key, reqs, ee, stream):
         gr = self.grid(key)
         s = gr.solver
         B = len(reqs)
@@ -149,6 +156,16 @@ class ServingEngine:
             ev = torch.cuda.Event()
             ev.record(stream)
         return {"ev": ev, "q": q, "status": status, "keep": (x, sc, n0, Tg, E, ns, ne)}
+
+    def _launch(self, key, reqs, ee, stream):
+        if not ee:
+            return self._solve(key, reqs, False, stream)
+
+        def work():
+            out = self._solve(key, reqs, True, stream)
+            out["ev"].synchronize()
+            return {k: v.cpu().numpy() for k, v in out["q"].items()}, out["status"].cpu().numpy()
+        return {"future": self._ee_pool.submit(work)}
 
     # ------------------------------------------------------------------ the loop
     def run(self, requests, realtime=True, poll=2e-4):
@@ -189,23 +206,35 @@ class ServingEngine:
             progressed = False
             # retire finished batches
             for bt in list(inflight):
-                if bt["ev"].query():
+                if "future" in bt:
+                    if not bt["future"].done():
+                        continue
+                    q, st = bt["future"].result()
+                elif bt["ev"].query():
                     q = {k: v.cpu().numpy() for k, v in bt["q"].items()}
                     st = bt["status"].cpu().numpy()
-                    done = time.perf_counter() - t0
-                    for j, r in enumerate(bt["reqs"]):
-                        results[r.rid] = Result(r.rid, float(q["energy"][j]), float(q["mobility"][j]),
-                                                float(q["diffusion"][j]), q["rates"][:, j].copy(), int(st[j]),
-                                                done - r.arrival, bt["t"], done, bt["id"])
-                    used -= bt["bytes"]
-                    inflight.remove(bt)
-                    progressed = True
+                else:
+                    continue
+                done = time.perf_counter() - t0
+                for j, r in enumerate(bt["reqs"]):
+                    results[r.rid] = Result(r.rid, float(q["energy"][j]), float(q["mobility"][j]),
+                                            float(q["diffusion"][j]), q["rates"][:, j].copy(), int(st[j]),
+                                            done - r.arrival, bt["t"], done, bt["id"])
+                used -= bt["bytes"]
+                inflight.remove(bt)
+                progressed = True
             # admit: asynchronous (no e-e) batches first, then e-e batches
             for want_ee in (False, True):
                 for qk in sorted(queues, key=lambda k: queues[k][0].arrival if queues[k] else 1e300):
                     qq = queues[qk]
                     if not qq or qk[3] != want_ee:
                         continue
+                    if want_ee:
+                        if any("future" in b for b in inflight):
+                            continue                           # one e-e batch at a time
+                        if (len(qq) < self.ee_min_batch and now - qq[0].arrival < self.ee_max_wait
+                                and i < len(reqs)):
+                            continue                           # let the batch grow
                     gr = self.grids[qk[:3]]
                     per = gr.bytes_per_instance(qk[3], self.restart)
                     nfit = min(len(qq), self.max_batch, (self.budget - used) // per)
@@ -222,11 +251,10 @@ class ServingEngine:
                     inflight.append(bt)
                     progressed = True
             if not progressed:
-                if realtime:
+                if realtime or any("future" in b for b in inflight):
                     time.sleep(poll)
                 elif inflight:
                     torch.cuda.synchronize()
-                    sim_now = now
         makespan = time.perf_counter() - t0
         lat = np.array([results[r.rid].latency for r in reqs])
         ok = np.array([results[r.rid].status in (0, 3) for r in reqs])
