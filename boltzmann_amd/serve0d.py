@@ -103,13 +103,13 @@ class _Grid:
 class ServingEngine:
     def __init__(self, xsec_file, hbm_budget, Te=4.0, ev_max=80.0, q=5, max_batch=4096, rtol=1e-12,
                  max_iter=30, ee_rtol=1e-15, ee_max_iter=60, restart=30, streams=4, ee_min_batch=256,
-                 ee_max_wait=0.05, device=None):
+                 ee_max_wait=0.05, floor_rel=1e-11, device=None):
         self.dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.xs = xsections_from_file(xsec_file, device=self.dev)
         self.Te, self.ev_max, self.q = Te, ev_max, q
         self.budget = int(hbm_budget)
         self.max_batch = int(max_batch)
-        self.rtol, self.max_iter = rtol, max_iter
+        self.rtol, self.max_iter, self.floor_rel = rtol, max_iter, floor_rel
         self.ee_rtol, self.ee_max_iter, self.restart = ee_rtol, ee_max_iter, restart
         self.grids = {}
         self.stats = {}
@@ -146,7 +146,10 @@ class ServingEngine:
             x = s.initial(B)
             sc = s.new_scratch(B)
             s.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=self.rtol, max_iter=self.max_iter, mode=1, scratch=sc)
-            status = sc["status"]
+            # an instance that ran out of iterations with |r| <= floor_rel |r0| sits at the FP64
+            # floor of its residual: reported as status 3, as the e-e solve does
+            status = torch.where((sc["status"] == 1) & (sc["res_abs"] <= self.floor_rel * sc["res0"]),
+                                 3, sc["status"]).to(torch.int32)
             if ee:
                 es, eop = gr.ee()
                 info = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, self.ee_rtol, self.ee_max_iter,
