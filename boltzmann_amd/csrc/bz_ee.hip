@@ -247,36 +247,64 @@ __global__ void k_ee_potentials_dd(const double* __restrict__ H, const double* _
 //   t2[b, (p,q)] = negam[b] sum_{(k,l),(r,s)} (cc_a[(p,q),(k,l),(r,s)] hv[(r,s), b]
 //                                              + cc_b[...] gv[(r,s), b]) x[(k,l), b]
 // x, hv, gv (n, B); t2 (B, n).  cc banded: |k - p|, |r - p| <= 3.
-__global__ void k_ee_t2(int ns, int npr, int B, const double* __restrict__ ccA, const double* __restrict__ ccB,
-                        const double* __restrict__ x, const double* __restrict__ hv, const double* __restrict__ gv,
-                        const double* __restrict__ negam, const int* __restrict__ active, double* __restrict__ t2) {
-  const int n = ns * npr, nc = ns * ns * ns;
-  const long total = (long)n * B;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const int inst = (int)(e % B), row = (int)(e / B);
-    if (active && !active[inst]) continue;
-    const int p = row / ns, q = row - p * ns;
-    double s = 0.0;
-    for (int dk = -3; dk <= 3; ++dk) {
-      const int k = p + dk;
+template <int NS>
+__global__ __launch_bounds__(256) void k_ee_t2(int npr, int B, const double* __restrict__ ccA,
+                                               const double* __restrict__ ccB, const double* __restrict__ x,
+                                               const double* __restrict__ hv, const double* __restrict__ gv,
+                                               const double* __restrict__ negam, const int* __restrict__ active,
+                                               double* __restrict__ t2) {
+  // one workgroup: radial row p, 256 instances; the tensor blocks of row p for one q at a
+  // time in LDS (read by every thread), the instance's potentials at r = p-3..p+3 in registers
+  constexpr int NC2 = NS * NS, NC = NS * NS * NS;
+  __shared__ double SA[49 * NC2], SB[49 * NC2];
+  const int p = blockIdx.y, tid = threadIdx.x;
+  const int inst = blockIdx.x * 256 + tid;
+  const int n = NS * npr;
+  const bool ok = inst < B && (!active || active[inst]);
+  double h[7][NS], g[7][NS];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    const int r = p + d - 3;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const bool in = ok && r >= 0 && r < npr;
+      h[d][s] = in ? hv[(long)(r * NS + s) * B + inst] : 0.0;
+      g[d][s] = in ? gv[(long)(r * NS + s) * B + inst] : 0.0;
+    }
+  }
+  const double ng = ok ? negam[inst] : 0.0;
+  for (int q = 0; q < NS; ++q) {
+    __syncthreads();
+    for (int e = tid; e < 49 * NC2; e += 256) {
+      const int dkdr = e / NC2, ls = e - dkdr * NC2;
+      const long src = ((long)p * 49 + dkdr) * NC + q * NC2 + ls;
+      SA[e] = ccA[src];
+      SB[e] = ccB[src];
+    }
+    __syncthreads();
+    if (!ok) continue;
+    double sum = 0.0;
+#pragma unroll
+    for (int dk = 0; dk < 7; ++dk) {
+      const int k = p + dk - 3;
       if (k < 0 || k >= npr) continue;
-      for (int l = 0; l < ns; ++l) {
-        const double xv = x[(long)(k * ns + l) * B + inst];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const double xv = x[(long)(k * NS + l) * B + inst];
         double a = 0.0;
-        for (int dr = -3; dr <= 3; ++dr) {
-          const int r = p + dr;
-          if (r < 0 || r >= npr) continue;
-          const long base = (((long)p * 7 + (dk + 3)) * 7 + (dr + 3)) * nc + (long)(q * ns + l) * ns;
-          for (int ss = 0; ss < ns; ++ss) {
-            const long pi = (long)(r * ns + ss) * B + inst;
-            a = fma(ccA[base + ss], hv[pi], a);
-            a = fma(ccB[base + ss], gv[pi], a);
+#pragma unroll
+        for (int dr = 0; dr < 7; ++dr) {
+#pragma unroll
+          for (int ss = 0; ss < NS; ++ss) {
+            const int e = (dk * 7 + dr) * NC2 + l * NS + ss;
+            a = fma(SA[e], h[dr][ss], a);
+            a = fma(SB[e], g[dr][ss], a);
           }
         }
-        s = fma(a, xv, s);
+        sum = fma(a, xv, sum);
       }
     }
-    t2[(long)inst * n + row] = negam[inst] * s;
+    t2[(long)inst * n + p * NS + q] = ng * sum;
   }
 }
 
@@ -373,9 +401,15 @@ int bz_ee_potentials_cols(const double* H, const double* G, int ns, int npr, con
 
 int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* ccB, const double* x, const double* hv,
                       const double* gv, const double* negam, const int* active, double* t2, hipStream_t stream) {
-  if (ns < 1 || npr < 1 || B < 1 || !ccA || !ccB || !x || !hv || !gv || !negam || !t2) return BZ_EINVAL;
-  hipLaunchKernelGGL(k_ee_t2, dim3(grid_for((long)ns * npr * B)), dim3(256), 0, stream, ns, npr, B, ccA, ccB, x, hv,
-                     gv, negam, active, t2);
+  if (ns < 1 || ns > 5 || npr < 1 || B < 1 || !ccA || !ccB || !x || !hv || !gv || !negam || !t2) return BZ_EINVAL;
+  const dim3 grid((B + 255) / 256, npr);
+  switch (ns) {
+    case 1: hipLaunchKernelGGL(k_ee_t2<1>, grid, dim3(256), 0, stream, npr, B, ccA, ccB, x, hv, gv, negam, active, t2); break;
+    case 2: hipLaunchKernelGGL(k_ee_t2<2>, grid, dim3(256), 0, stream, npr, B, ccA, ccB, x, hv, gv, negam, active, t2); break;
+    case 3: hipLaunchKernelGGL(k_ee_t2<3>, grid, dim3(256), 0, stream, npr, B, ccA, ccB, x, hv, gv, negam, active, t2); break;
+    case 4: hipLaunchKernelGGL(k_ee_t2<4>, grid, dim3(256), 0, stream, npr, B, ccA, ccB, x, hv, gv, negam, active, t2); break;
+    default: hipLaunchKernelGGL(k_ee_t2<5>, grid, dim3(256), 0, stream, npr, B, ccA, ccB, x, hv, gv, negam, active, t2);
+  }
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }
