@@ -878,12 +878,26 @@ __global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, 
     }
     const double* S = SB + st * LBUF;
     const int base = PO[r0];
-    for (int i = r0; i < r1; ++i) {
-      const int fc = FC[i], off = PO[i] - base - fc;
-      double sacc = 0.0;
-      for (int j = fc + lane; j < i; j += 64) sacc = fma(S[off + j], Y2[j], sacc);
-      sacc = wave_sum(sacc);
-      if (lane == 0) Y2[i] -= sacc;
+    // blocks of 64 rows, lane t owning row blk0 + t: first the columns left of the block
+    // (values final), then the unit lower triangle of the block, one column per step with
+    // the finished value broadcast from its lane
+    for (int blk0 = r0; blk0 < r1; blk0 += 64) {
+      const int nb = min(64, r1 - blk0);
+      const int i = blk0 + lane;
+      const bool act = lane < nb;
+      const int fc = act ? (int)FC[i] : 0;
+      const int off = act ? PO[i] - base - fc : 0;
+      double acc = act ? Y2[i] : 0.0;
+      if (act)
+        for (int j = fc; j < blk0; ++j) acc = fma(-S[off + j], Y2[j], acc);
+      for (int t = 0; t < nb; ++t) {
+        const int j = blk0 + t;
+        const int lo = __builtin_amdgcn_readlane(__double2loint(acc), t);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(acc), t);
+        const double yj = __hiloint2double(hi, lo);
+        if (act && i > j && j >= fc) acc = fma(-S[off + j], yj, acc);
+      }
+      if (act) Y2[i] = acc;
       __syncthreads();
     }
     double* D = SB + (st ^ 1) * LBUF;
