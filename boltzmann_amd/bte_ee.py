@@ -237,7 +237,7 @@ class EEOperator:
                 Bw = nt
                 todo = (w.fn > tolw) & ~failed[idx] & ~floor[idx]
             sweeps += 1
-            iters[idx] += todo.to(i32)
+            iters.index_add_(0, idx, todo.to(i32))
             # ---- GMRES on J d = -F, right-preconditioned by the frozen-potential factorisation
             w.active.copy_(todo.to(i32))
             _lib.call("bz_ee_newton_factor", grid, bt, w.active, w.lprof, w.z1, w.s1)
@@ -256,7 +256,7 @@ class EEOperator:
                           w.kdim, w.resid)
                 if not bool(w.active.any()):
                     break
-            krylov[idx] += w.kdim
+            krylov.index_add_(0, idx, w.kdim)
             _lib.call("bz_gmres_combine_batched", n, Bw, m, w.V, w.R, w.g, w.kdim, w.Jz)
             w.active.copy_((w.kdim > 0).to(i32))
             w.z.zero_()
@@ -274,15 +274,15 @@ class EEOperator:
             torch.addcmul(xw, w.zT, alpha[None, :], out=x_try)
             bt, negam, keep = evaluate(x_try, w, prm)
             ok = w.fn * w.fn <= fn0 * fn0 * (1.0 - 2.0 * line_search_c * alpha)
-            xw[:, pending & ok] = x_try[:, pending & ok]
+            torch.where((pending & ok)[None, :], x_try, xw, out=xw)
             pending &= ~ok
             while bool(pending.any()):
                 accepted_first = False
                 alpha = torch.where(pending, alpha * alpha_rho, alpha)
                 flat = pending & (alpha < floor_alpha) & (fn0 <= floor_rel * r0w)
                 dead = pending & (alpha < alpha_min)
-                floor[idx[flat]] = True
-                failed[idx[dead]] = True
+                floor.index_copy_(0, idx, floor[idx] | flat)
+                failed.index_copy_(0, idx, failed[idx] | dead)
                 stopped |= flat | dead
                 pending &= ~(dead | flat)
                 pid = pending.nonzero().squeeze(1)
