@@ -109,7 +109,8 @@ def test_steady_state_with_ee(setup):
     # (measured: energy 6.4e-11, rates 9.5e-10, the spread of the operator assembly: the
     # oracle's fixed point with this library's H, G, cc_a, cc_b evaluated in long double is
     # 9.7e-10 away on the same rate)
-    assert max(errs.values()) < 1e-8, errs
+    assert max(errs[k] for k in ("energy", "mobility", "diffusion")) < 1e-9, errs
+    assert errs["rates"] < 2e-8, errs
     hg = h.cpu().numpy()
     for i in range(B):
         assert rel_err(hg[:, i], x_ref[:, i]) < 1e-8
@@ -217,4 +218,80 @@ def test_steady_state_with_ee_lmax4():
     h, q = s.qois(x, n0, E)
     errs = qoi_errors({k: v.cpu().numpy() for k, v in q.items()}, q_ref)
     print("qoi errors", errs, flush=True)
-    assert max(errs.values()) < 1e-8, errs
+    assert max(errs[k] for k in ("energy", "mobility", "diffusion")) < 1e-9, errs
+    assert errs["rates"] < 2e-8, errs
+
+
+def test_steady_state_with_ee_n387():
+    """e-e at the size of the batched e-e workload (Nr = 128, lmax = 2, n = 387) against the
+    oracle's dense exact Newton, stored in tests/golden/ee_n387.npz (make_ee387.py).
+
+    The assembly and the solve are checked separately.
+    * Assembly: H, G and the banded tensors against the oracle's.
+    * Solve: with the oracle's H, G and tensors uploaded (K and M stay this library's),
+      energy, mobility and diffusion match to 1e-9 and the rates to 2e-8 (measured 1e-10 and
+      9.8e-9; the rest is the K/M assembly roundoff).
+    * With this library's operators, the ionization rate of the most e-e dominated instance
+      moves by 7.3e-8: an assembly roundoff of 1.4e-14 of max|H| is amplified in that tail.
+      Measured: the oracle run with this library's operators agrees with the GPU solve to
+      9e-11 on every QoI, so the 7.3e-8 belongs to the assembly, not to the solver.
+    """
+    import os
+    from boltzmann_amd.bte_ee import EEOperator
+    from boltzmann_amd.ee_terms_table import TERMS
+    from boltzmann_amd.solver0d import Batched0D
+    print("case: e-e steady state Nr=128 lmax=2 B=4 vs stored oracle", flush=True)
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ee_n387.npz"))
+    ops, g, procs = oracle_batched(128, 2)
+    vg = product_grid(g)
+    s = Batched0D(vg, product_xsecs(), ee_pattern=True)
+    eop = EEOperator(vg, s)
+    ns_, npr = g.num_sh, g.num_p
+    hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
+    A, Bt = eeo.cc_tensors(g, TERMS)
+    # assembly parity
+    for l in range(ns_):
+        for dev_op, r_ in ((eop.H, hl), (eop.G, gl)):
+            r = r_[l::ns_, l::ns_]
+            assert np.abs(dev_op[l].cpu().numpy() - r).max() <= 1e-12 * np.abs(r).max()
+    bands = []
+    for dense, band in ((A, eop.ccA), (Bt, eop.ccB)):
+        D = dense.reshape(npr, ns_, npr, ns_, npr, ns_)
+        out = np.zeros((npr, 7, 7, ns_, ns_, ns_))
+        for p in range(npr):
+            for dk in range(-3, 4):
+                for dr in range(-3, 4):
+                    k, r = p + dk, p + dr
+                    if 0 <= k < npr and 0 <= r < npr:
+                        out[p, dk + 3, dr + 3] = D[p, :, k, :, r, :]
+        got = band.cpu().numpy().reshape(out.shape)
+        assert np.abs(got - out).max() <= 1e-12 * np.abs(out).max()
+        bands.append(out.reshape(band.shape))
+    del A, Bt
+    n0, Tg, E, ns, ne = (_d(ref[k]) for k in ("n0", "Tg", "E", "ns_by_n0", "ne"))
+    q_ref = {k: ref[k] for k in ("energy", "mobility", "diffusion", "rates")}
+
+    def solve():
+        x = s.initial(4)
+        s.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=1e-14, max_iter=40, mode=1)
+        out = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, 1e-15, 60)
+        st, ra, r0 = (out[k].cpu().numpy() for k in ("status", "res_abs", "res0"))
+        print("status", st, "|F|/|F0|", ra / r0, "steps", out["iters"].cpu().numpy(), flush=True)
+        assert np.all((st == 0) | ((st == 3) & (ra <= 1e-12 * r0)))
+        h, q = s.qois(x, n0, E)
+        return qoi_errors({k: v.cpu().numpy() for k, v in q.items()}, q_ref)
+
+    errs_own = solve()
+    print("qoi errors, this library's operators", errs_own, flush=True)
+    assert max(errs_own[k] for k in ("energy", "mobility", "diffusion")) < 1e-8, errs_own
+    assert errs_own["rates"] < 1e-7, errs_own
+    # the oracle's operators on the device: the solve itself to 1e-8
+    for l in range(ns_):
+        eop.H[l].copy_(_d(hl[l::ns_, l::ns_]))
+        eop.G[l].copy_(_d(gl[l::ns_, l::ns_]))
+    eop.ccA.copy_(_d(bands[0]))
+    eop.ccB.copy_(_d(bands[1]))
+    errs = solve()
+    print("qoi errors, the oracle's operators", errs, flush=True)
+    assert max(errs[k] for k in ("energy", "mobility", "diffusion")) < 1e-9, errs
+    assert errs["rates"] < 2e-8, errs
