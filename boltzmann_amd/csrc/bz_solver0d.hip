@@ -201,21 +201,42 @@ __device__ void mass_matvec(int n, int ns, const double* __restrict__ mprof, con
   }
 }
 
-// Register state of the factorisation: R levels of row slots per lane.
+// Register state of the factorisation: R levels of row slots per lane.  Each slot also
+// holds the row that takes it next (nrow), so that one LDS read of that row's entry gives
+// all a slot change needs.  b1/b2 hold the right-hand-side corrections -sum_j m_rj y_j
+// accumulated since the row entered; its entry value is added at its pivot step.
 template <int W, int R>
 struct FactorState {
-  int row[R], fc[R], lc[R], off[R];
+  int row[R], fc[R], off[R], nrow[R];
   unsigned vmask[R];
   double w[R][W], nk[R][W], b1[R], b2[R];
 };
+
+// Row table, one int2 per row i: x = first profile column of i (low 16 bits) | (next row
+// in i's slot << 16, -1 for none), y = profile offset of i minus that column (entries of
+// the row are kprof[y + c]).
+__device__ __forceinline__ void fill_rowtab(const GridDev& g, int2* __restrict__ rt, int lane) {
+  for (int i = lane; i < g.n; i += 64) {
+    const int fc = g.first_col[i];
+    int2 v;
+    v.x = (int)((unsigned)(fc & 0xffff) | ((unsigned)g.sched_next[i] << 16));
+    v.y = g.prof_off[i] - fc;
+    rt[i] = v;
+  }
+}
+
+// Slot state for row `row` (row < 0: an empty slot, first column INT_MAX, never valid).
+__device__ __forceinline__ void row_enter(int2 v, int row, int& fc, int& off, int& nrow) {
+  fc = (row >= 0) ? (v.x & 0xffff) : 0x7fffffff;
+  off = (row >= 0) ? v.y : 0;
+  nrow = (row >= 0) ? (v.x >> 16) : -1;
+}
 
 // One W-step chunk of the factorisation running levels 0..RA-1 only.  SL: also store the
 // multipliers l_rj of L at the profile positions of (r, j) in lout.
 template <int W, int R, int RA, bool SL>
 __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
-                                             const double* __restrict__ kprof,
-                                             const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
-                                             const short* __restrict__ s_next, const double* __restrict__ muhat,
+                                             const double* __restrict__ kprof, const int2* __restrict__ s_succ,
                                              double* __restrict__ Y1, double* __restrict__ Y2,
                                              double* __restrict__ s_piv, double* __restrict__ urows, int WP,
                                              int lane, double* __restrict__ lout) {
@@ -223,6 +244,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
     if (j >= n) break;
+    const double e1 = Y1[j], e2 = Y2[j];     // entry values of row j's right-hand sides
     // --- locate the pivot row j
     int hit = -1;
 #pragma unroll
@@ -248,7 +270,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
     double pv[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
-    const double pb1 = s_piv[W], pb2 = s_piv[W + 1];
+    const double pb1 = e1 + s_piv[W], pb2 = e2 + s_piv[W + 1];
     const double rinv = pv[0];
     // --- U row j and the eliminated rhs; every lane stores entry lane % WP (no branch)
     {
@@ -268,25 +290,19 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
       s.b2[r] = fma(-m, pb2, s.b2[r]);
       s.w[r][t] = ((s.vmask[r] >> t) & 1u) ? s.nk[r][t] : 0.0;
     }
-    // --- retire the pivot row; its slot takes the next scheduled row (LDS reads only)
+    // --- retire the pivot row; its slot takes the next scheduled row (one LDS read)
     if (lane == lp) {
 #pragma unroll
       for (int r = 0; r < RA; ++r) {
         if (hit == r) {
-          const int nr = s_next[s.row[r]];
+          const int nr = s.nrow[r];
           s.row[r] = nr;
           s.vmask[r] = 0u;
+          s.b1[r] = 0.0;
+          s.b2[r] = 0.0;
 #pragma unroll
           for (int k = 0; k < W; ++k) s.w[r][k] = 0.0;
-          if (nr >= 0) {
-            s.fc[r] = s_fcol[nr];
-            s.lc[r] = min(nr + bw, n - 1);
-            s.off[r] = s_poff[nr] - s.fc[r];
-            s.b1[r] = muhat[nr];
-            s.b2[r] = Y2[nr];
-          } else {
-            s.fc[r] = 0x7fffffff; s.lc[r] = -1; s.off[r] = 0; s.b1[r] = 0.0; s.b2[r] = 0.0;
-          }
+          row_enter(s_succ[nr >= 0 ? nr : 0], nr, s.fc[r], s.off[r], s.nrow[r]);
         }
       }
     }
@@ -294,7 +310,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int c = j + 2 * W;
-      const bool valid = (c >= s.fc[r]) && (c <= s.lc[r]);
+      const bool valid = (c >= s.fc[r]) && (c <= min(s.row[r] + bw, n - 1));
       const int idx = valid ? s.off[r] + c : 0;
       s.nk[r][t] = kprof[idx];
       s.vmask[r] = (s.vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
@@ -311,8 +327,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 // a fixed number of loads and the memory-counter waits can be counted statically.
 template <int W, int R, bool SL = false>
 __device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof,
-                               const short* __restrict__ s_fcol, const int* __restrict__ s_poff,
-                               const short* __restrict__ s_next, const short* __restrict__ s_lev,
+                               const int2* __restrict__ s_succ, const short* __restrict__ s_lev,
                                double* __restrict__ Y1, double* __restrict__ Y2, double* __restrict__ s_piv,
                                double* __restrict__ urows, int WP, int lane, double* __restrict__ lout = nullptr) {
   const int n = g.n, bw = g.bw;
@@ -321,38 +336,31 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   for (int r = 0; r < R; ++r) {
     const int rw = g.slot_first[r * 64 + lane];
     s.row[r] = rw;
-    if (rw >= 0) {
-      s.fc[r] = s_fcol[rw];
-      s.lc[r] = min(rw + bw, n - 1);
-      s.off[r] = s_poff[rw] - s.fc[r];
-      s.b1[r] = Y1[rw];
-      s.b2[r] = Y2[rw];
-    } else {
-      s.fc[r] = 0x7fffffff; s.lc[r] = -1; s.off[r] = 0; s.b1[r] = 0.0; s.b2[r] = 0.0;
-    }
+    const int lc = (rw >= 0) ? min(rw + bw, n - 1) : -1;
+    row_enter(s_succ[rw >= 0 ? rw : 0], rw, s.fc[r], s.off[r], s.nrow[r]);
+    s.b1[r] = 0.0;
+    s.b2[r] = 0.0;
     s.vmask[r] = 0u;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       const int c0 = k, c1 = W + k;
       double v0 = 0.0, a1 = 0.0;
-      if (c0 >= s.fc[r] && c0 <= s.lc[r]) v0 = kprof[s.off[r] + c0];
-      if (c1 >= s.fc[r] && c1 <= s.lc[r]) {
+      if (c0 >= s.fc[r] && c0 <= lc) v0 = kprof[s.off[r] + c0];
+      if (c1 >= s.fc[r] && c1 <= lc) {
         a1 = kprof[s.off[r] + c1];
         s.vmask[r] |= 1u << k;
       }
       s.w[r][k] = v0; s.nk[r][k] = a1;
     }
   }
-  // Y1/Y2 double as the entry values of the rhs (M uhat, M x): a row entering its
-  // slot during step j has index > j, and Y*[i] is only overwritten at pivot step i.
+  // Y1/Y2 hold the entry values of the rhs (M uhat, M x) until pivot step i overwrites
+  // Y*[i] with the eliminated value; step j reads Y*[j] before it writes it.
   for (int j0 = 0, ch = 0; j0 < n; j0 += W, ++ch) {
     const int levels = __builtin_amdgcn_readfirstlane((int)s_lev[ch]);
     if (R == 1 || levels <= 1) {
-      factor_chunk<W, R, 1, SL>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane,
-                                lout);
+      factor_chunk<W, R, 1, SL>(j0, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
     } else {
-      factor_chunk<W, R, R, SL>(j0, n, bw, s, kprof, s_fcol, s_poff, s_next, Y1, Y1, Y2, s_piv, urows, WP, lane,
-                                lout);
+      factor_chunk<W, R, R, SL>(j0, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
     }
   }
 }
@@ -464,7 +472,8 @@ __global__ void k_prepare0d(GridDev g, BatchDev b) {
 }
 
 // LDS layout (doubles first): X, Z1, Z2 (+ pad so Z1..Z2 can hold 4*npr mass Cholesky
-// coefficients), AV, STG[2W*WP], PIV[PIVN]; then PO (int), FC, NX, LEV (short).
+// coefficients), AV, STG[2W*WP], PIV[PIVN]; then SC (int2, row table), PO (int),
+// LEV (short).
 __host__ __device__ __forceinline__ int zpad(int n, int npr) { return (4 * npr > 2 * n) ? (4 * npr - 2 * n) : 0; }
 
 // Two waves per SIMD (register budget 256 incl. spills) measured faster than one
@@ -482,10 +491,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   double* AV = Z2 + n + zpad(n, g.npr);                    // M^-1 K x
   double* STG = AV + n;                                    // back-substitution stage
   double* PIV = STG + 2 * W * WP;                          // published pivot row
-  int* PO = reinterpret_cast<int*>(PIV + PIVN);
-  short* FC = reinterpret_cast<short*>(PO + (n + 1));
-  short* NX = FC + n;
-  short* LEV = NX + n;
+  int2* SC = reinterpret_cast<int2*>(PIV + PIVN);
+  int* PO = reinterpret_cast<int*>(SC + n);
+  short* LEV = reinterpret_cast<short*>(PO + (n + 1));
   double* LCH = Z1;                                        // staged only when Z1/Z2 are free
 
   double* kprof = b.kprof + (size_t)inst * g.P;           // holds K - mu_app M
@@ -496,11 +504,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   double* urows = b.urows + (size_t)inst * n * WP;
 
   // ---------------- tables into LDS, initial iterate (original (n,B) layout -> permuted)
-  for (int i = lane; i < n; i += 64) {
-    FC[i] = (short)g.first_col[i];
-    NX[i] = (short)g.sched_next[i];
-    X[i] = b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst];
-  }
+  fill_rowtab(g, SC, lane);
+  for (int i = lane; i < n; i += 64) X[i] = b.x[(size_t)orig_index(i, g.ns, g.npr) * b.B + inst];
   for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
   for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
   __syncthreads();
@@ -566,7 +571,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int i = k / 7, dd = k - 7 * (k / 7) - 3;
         const int c = i + dd * g.ns;
         if (c >= 0 && c < n) {
-          const int e = PO[i] + c - FC[i];
+          const int e = PO[i] + c - g.first_col[i];
           kprof[e] = fma(-dmu, mprof[e], kprof[e]);
         }
       }
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
     mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, X, Z2, lane);
     __syncthreads();
-    factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
+    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane);
     __syncthreads();
     back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
@@ -638,8 +643,8 @@ size_t newton_lds_bytes(int n, int npr) {
   constexpr int WP = (W + 1) & ~1;
   constexpr int PIVN = (W + 3) & ~1;
   const int nchunk = (n + W - 1) / W;
-  return (size_t)(4 * n + zpad(n, npr) + 2 * W * WP + PIVN) * sizeof(double) + (size_t)(n + 1) * sizeof(int) +
-         (size_t)(2 * n + nchunk) * sizeof(short);
+  return (size_t)(4 * n + zpad(n, npr) + 2 * W * WP + PIVN) * sizeof(double) + (size_t)n * sizeof(int2) +
+         (size_t)(n + 1) * sizeof(int) + (size_t)nchunk * sizeof(short);
 }
 
 struct EEDev {
@@ -791,22 +796,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   double* AV = Z2 + n + zpad(n, g.npr);
   double* STG = AV + n;
   double* PIV = STG + 2 * W * WP;
-  int* PO = reinterpret_cast<int*>(PIV + PIVN);
-  short* FC = reinterpret_cast<short*>(PO + (n + 1));
-  short* NX = FC + n;
-  short* LEV = NX + n;
+  int2* SC = reinterpret_cast<int2*>(PIV + PIVN);
+  short* LEV = reinterpret_cast<short*>(SC + n);
   const double* kprof = b.kprof + (size_t)inst * g.P;
   double* urows = b.urows + (size_t)inst * n * WP;
+  fill_rowtab(g, SC, lane);
   for (int i = lane; i < n; i += 64) {
-    FC[i] = (short)g.first_col[i];
-    NX[i] = (short)g.sched_next[i];
     Z1[i] = g.muhat[i];
     Z2[i] = 0.0;
   }
-  for (int i = lane; i <= n; i += 64) PO[i] = g.prof_off[i];
   for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
   __syncthreads();
-  factor_forward<W, R, true>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane, lprof + (size_t)inst * g.P);
+  factor_forward<W, R, true>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane, lprof + (size_t)inst * g.P);
   __syncthreads();
   back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
   __syncthreads();
@@ -992,10 +993,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   double* STG = AV + n;
   double* PIV = STG + 2 * W * WP;
   double* U0 = PIV + PIVN;
-  int* PO = reinterpret_cast<int*>(U0 + n);
-  short* FC = reinterpret_cast<short*>(PO + (n + 1));
-  short* NX = FC + n;
-  short* LEV = NX + n;
+  int2* SC = reinterpret_cast<int2*>(U0 + n);
+  int* PO = reinterpret_cast<int*>(SC + n);
+  short* LEV = reinterpret_cast<short*>(PO + (n + 1));
   double* LCH = Z1;
 
   double* kprof = b.kprof + (size_t)inst * g.P;
@@ -1004,9 +1004,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const double* __restrict__ WV = b.vec + (size_t)inst * 4 * n;
   double* urows = b.urows + (size_t)inst * n * WP;
 
+  fill_rowtab(g, SC, lane);
   for (int i = lane; i < n; i += 64) {
-    FC[i] = (short)g.first_col[i];
-    NX[i] = (short)g.sched_next[i];
     const size_t o = (size_t)orig_index(i, g.ns, g.npr) * b.B + inst;
     X[i] = b.x[o];                              // initial iterate
     U0[i] = ustart ? ustart[o] : b.x[o];        // state at the start of the step
@@ -1022,7 +1021,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int i = k / 7, dd = k - 7 * (k / 7) - 3;
     const int c = i + dd * g.ns;
     if (c >= 0 && c < n) {
-      const int e = PO[i] + c - FC[i];
+      const int e = PO[i] + c - g.first_col[i];
       kprof[e] = fma(-mup, mprof[e], kprof[e]);
     }
   }
@@ -1069,7 +1068,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
     mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, AV, Z2, lane);
     __syncthreads();
-    factor_forward<W, R>(g, kprof, FC, PO, NX, LEV, Z1, Z2, PIV, urows, WP, lane);
+    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane);
     __syncthreads();
     back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
