@@ -233,8 +233,11 @@ __device__ __forceinline__ void row_enter(int2 v, int row, int& fc, int& off, in
 }
 
 // One W-step chunk of the factorisation running levels 0..RA-1 only.  SL: also store the
-// multipliers l_rj of L at the profile positions of (r, j) in lout.
-template <int W, int R, int RA, bool SL>
+// multipliers l_rj of L at the profile positions of (r, j) in lout.  TAIL: the chunk may
+// end past row n-1 (only the last one); full chunks carry no exit test, so the compiler
+// places no spill reloads for an exit path inside the steps (a reload waits on the
+// in-order memory counter, i.e. on every column prefetch issued before it).
+template <int W, int R, int RA, bool SL, bool TAIL>
 __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
                                              const double* __restrict__ kprof, const int2* __restrict__ s_succ,
                                              double* __restrict__ Y1, double* __restrict__ Y2,
@@ -243,7 +246,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 #pragma unroll
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
-    if (j >= n) break;
+    if (TAIL && j >= n) break;
     const double e1 = Y1[j], e2 = Y2[j];     // entry values of row j's right-hand sides
     // --- locate the pivot row j
     int hit = -1;
@@ -355,14 +358,18 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   }
   // Y1/Y2 hold the entry values of the rhs (M uhat, M x) until pivot step i overwrites
   // Y*[i] with the eliminated value; step j reads Y*[j] before it writes it.
-  for (int j0 = 0, ch = 0; j0 < n; j0 += W, ++ch) {
+  const int nfull = n / W;
+  for (int ch = 0; ch < nfull; ++ch) {
     const int levels = __builtin_amdgcn_readfirstlane((int)s_lev[ch]);
     if (R == 1 || levels <= 1) {
-      factor_chunk<W, R, 1, SL>(j0, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
+      factor_chunk<W, R, 1, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
     } else {
-      factor_chunk<W, R, R, SL>(j0, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
+      factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
     }
   }
+  // last partial chunk: all levels (running a level with no rows is harmless)
+  if (nfull * W < n)
+    factor_chunk<W, R, R, SL, true>(nfull * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
 }
 
 // Back substitution U z = y for the two right-hand sides held in Y1/Y2 (in place).
