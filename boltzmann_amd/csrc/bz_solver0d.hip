@@ -356,17 +356,15 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
       s.w[r][k] = v0; s.nk[r][k] = a1;
     }
   }
+  // Wait for the initial window loads here, once: with them still in flight at the loop
+  // entry, the compiler's wait analysis would put a full vector-memory wait in every chunk.
+  // (gfx9 s_waitcnt fields: vmcnt 0, expcnt 7, lgkmcnt 15.)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   // Y1/Y2 hold the entry values of the rhs (M uhat, M x) until pivot step i overwrites
   // Y*[i] with the eliminated value; step j reads Y*[j] before it writes it.
   const int nfull = n / W;
-  for (int ch = 0; ch < nfull; ++ch) {
-    const int levels = __builtin_amdgcn_readfirstlane((int)s_lev[ch]);
-    if (R == 1 || levels <= 1) {
-      factor_chunk<W, R, 1, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
-    } else {
-      factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
-    }
-  }
+  for (int ch = 0; ch < nfull; ++ch)
+    factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
   // last partial chunk: all levels (running a level with no rows is harmless)
   if (nfull * W < n)
     factor_chunk<W, R, R, SL, true>(nfull * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
