@@ -282,10 +282,12 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
       Y1[j] = pb1;
       Y2[j] = pb2;
     }
-    // --- eliminate column j, shift in column j+W (prefetched 2W steps ago)
+    // --- eliminate column j (rows whose first column lies beyond j have a zero there: their
+    // window may still hold the previous row of the slot), shift in column j+W (prefetched
+    // W steps ago)
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
-      const double m = s.w[r][t] * rinv;
+      const double m = (j >= s.fc[r]) ? s.w[r][t] * rinv : 0.0;
       if (SL && s.row[r] > j && j >= s.fc[r]) lout[s.off[r] + j] = m;
 #pragma unroll
       for (int k = 1; k < W; ++k) s.w[r][(t + k) % W] = fma(-m, pv[k], s.w[r][(t + k) % W]);
@@ -293,7 +295,9 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
       s.b2[r] = fma(-m, pb2, s.b2[r]);
       s.w[r][t] = ((s.vmask[r] >> t) & 1u) ? s.nk[r][t] : 0.0;
     }
-    // --- retire the pivot row; its slot takes the next scheduled row (one LDS read)
+    // --- retire the pivot row; its slot takes the next scheduled row (one LDS read).  Its
+    // window is not cleared: the shift-ins of the next W steps overwrite every position
+    // (with zero, vmask being clear), and the row's first column is at least 2W steps away.
     if (lane == lp) {
 #pragma unroll
       for (int r = 0; r < RA; ++r) {
@@ -303,19 +307,18 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
           s.vmask[r] = 0u;
           s.b1[r] = 0.0;
           s.b2[r] = 0.0;
-#pragma unroll
-          for (int k = 0; k < W; ++k) s.w[r][k] = 0.0;
           row_enter(s_succ[nr >= 0 ? nr : 0], nr, s.fc[r], s.off[r], s.nrow[r]);
         }
       }
     }
-    // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked)
+    // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked);
+    // a 32-bit byte offset from the instance's profile base
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int c = j + 2 * W;
       const bool valid = (c >= s.fc[r]) && (c <= min(s.row[r] + bw, n - 1));
-      const int idx = valid ? s.off[r] + c : 0;
-      s.nk[r][t] = kprof[idx];
+      const unsigned boff = valid ? ((unsigned)(s.off[r] + c) << 3) : 0u;
+      s.nk[r][t] = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(kprof) + boff);
       s.vmask[r] = (s.vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
     }
   }
