@@ -241,29 +241,36 @@ template <int W, int R, int RA, bool SL, bool TAIL>
 __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
                                              const double* __restrict__ kprof, const int2* __restrict__ s_succ,
                                              double* __restrict__ Y1, double* __restrict__ Y2,
-                                             double* __restrict__ s_piv, double* __restrict__ urows, int WP,
-                                             int lane, double* __restrict__ lout) {
+                                             double* __restrict__ s_piv, double* __restrict__ s_junk,
+                                             double* __restrict__ urows, int WP, int lane,
+                                             double* __restrict__ lout) {
 #pragma unroll
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
     if (TAIL && j >= n) break;
     const double e1 = Y1[j], e2 = Y2[j];     // entry values of row j's right-hand sides
-    // --- locate the pivot row j
-    int hit = -1;
+    // --- locate the pivot row j: lane lp, level rr (both wave-uniform)
+    int lp = 0, rr = 0;
 #pragma unroll
-    for (int r = 0; r < RA; ++r) hit = (s.row[r] == j) ? r : hit;
-    const unsigned long long bal = __ballot(hit >= 0);
-    const int lp = __builtin_amdgcn_readfirstlane(__ffsll((long long)bal) - 1);
-    // --- pivot lane publishes 1/U_jj, U_j,j+1..j+W-1 and the two rhs values
-    if (lane == lp) {
+    for (int r = 0; r < RA; ++r) {
+      const unsigned long long bl = __ballot(s.row[r] == j);
+      if (bl) { lp = __ffsll((long long)bl) - 1; rr = r; }
+    }
+    lp = __builtin_amdgcn_readfirstlane(lp);
+    rr = __builtin_amdgcn_readfirstlane(rr);
+    const int hit = (lane == lp) ? rr : -1;
+    // --- publish U_jj..U_j,j+W-1 and the two rhs values.  Every lane stores (no divergent
+    // branch): level rr of the pivot lane into s_piv, the others into their own row of
+    // s_junk.
+    {
+      double* dst = (lane == lp) ? s_piv : s_junk + lane;
 #pragma unroll
       for (int r = 0; r < RA; ++r) {
-        if (hit == r) {
-          s_piv[0] = fast_rcp(s.w[r][t]);
+        if (rr == r) {
 #pragma unroll
-          for (int k = 1; k < W; ++k) s_piv[k] = s.w[r][(t + k) % W];
-          s_piv[W] = s.b1[r];
-          s_piv[W + 1] = s.b2[r];
+          for (int k = 0; k < W; ++k) dst[k] = s.w[r][(t + k) % W];
+          dst[W] = s.b1[r];
+          dst[W + 1] = s.b2[r];
         }
       }
     }
@@ -274,11 +281,11 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 #pragma unroll
     for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
     const double pb1 = e1 + s_piv[W], pb2 = e2 + s_piv[W + 1];
-    const double rinv = pv[0];
-    // --- U row j and the eliminated rhs; every lane stores entry lane % WP (no branch)
+    const double rinv = fast_rcp(pv[0]);
+    // --- U row j (1/U_jj first) and the eliminated rhs; every lane stores entry lane % WP
     {
       const int kk = lane % WP;
-      urows[(size_t)j * WP + kk] = s_piv[kk];
+      urows[(size_t)j * WP + kk] = (kk == 0) ? rinv : s_piv[kk];
       Y1[j] = pb1;
       Y2[j] = pb2;
     }
@@ -335,7 +342,8 @@ template <int W, int R, bool SL = false>
 __device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof,
                                const int2* __restrict__ s_succ, const short* __restrict__ s_lev,
                                double* __restrict__ Y1, double* __restrict__ Y2, double* __restrict__ s_piv,
-                               double* __restrict__ urows, int WP, int lane, double* __restrict__ lout = nullptr) {
+                               double* __restrict__ s_junk, double* __restrict__ urows, int WP, int lane,
+                               double* __restrict__ lout = nullptr) {
   const int n = g.n, bw = g.bw;
   FactorState<W, R> s;
 #pragma unroll
@@ -367,10 +375,12 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   // Y*[i] with the eliminated value; step j reads Y*[j] before it writes it.
   const int nfull = n / W;
   for (int ch = 0; ch < nfull; ++ch)
-    factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
+    factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, s_junk, urows, WP, lane,
+                                       lout);
   // last partial chunk: all levels (running a level with no rows is harmless)
   if (nfull * W < n)
-    factor_chunk<W, R, R, SL, true>(nfull * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, urows, WP, lane, lout);
+    factor_chunk<W, R, R, SL, true>(nfull * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, s_junk, urows, WP, lane,
+                                     lout);
 }
 
 // Back substitution U z = y for the two right-hand sides held in Y1/Y2 (in place).
@@ -588,7 +598,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
     mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, X, Z2, lane);
     __syncthreads();
-    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane);
+    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, STG, urows, WP, lane);
     __syncthreads();
     back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
@@ -815,7 +825,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
   for (int i = lane; i < nchunk; i += 64) LEV[i] = (short)g.chunk_levels[i];
   __syncthreads();
-  factor_forward<W, R, true>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane, lprof + (size_t)inst * g.P);
+  factor_forward<W, R, true>(g, kprof, SC, LEV, Z1, Z2, PIV, STG, urows, WP, lane, lprof + (size_t)inst * g.P);
   __syncthreads();
   back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
   __syncthreads();
@@ -1076,7 +1086,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
     mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, AV, Z2, lane);
     __syncthreads();
-    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, urows, WP, lane);
+    factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, STG, urows, WP, lane);
     __syncthreads();
     back_substitute<W>(n, urows, WP, Z1, Z2, STG, lane);
     __syncthreads();
