@@ -582,21 +582,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     if (b.mode == 0) abs_err = rcur;
     mu = dot(WV, X, n, lane);
     {
-      // shift the profile from K - mu_app M to K - mu M on the band of M
+      // one pass over the rows: shift the profile from K - mu_app M to K - mu M on the band
+      // of M (row i: columns i + d*ns, |d| <= 3) and form the right-hand sides M uhat, M x.
+      // The row table gives the row's profile offset minus its first column; all band values
+      // of a row are loaded before any store, so they are in flight together.
       const double dmu = mu - mu_app;
-      const int nm = 7 * n;
-      for (int k = lane; k < nm; k += 64) {
-        const int i = k / 7, dd = k - 7 * (k / 7) - 3;
-        const int c = i + dd * g.ns;
-        if (c >= 0 && c < n) {
-          const int e = PO[i] + c - g.first_col[i];
-          kprof[e] = fma(-dmu, mprof[e], kprof[e]);
+      for (int i = lane; i < n; i += 64) {
+        const int base = SC[i].y;
+        double kv[7], mv[7];
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+          const int c = i + (d - 3) * g.ns;
+          const bool ok = (c >= 0) && (c < n);
+          const int cc = ok ? c : i;                 // the diagonal: always in the row
+          mv[d] = ok ? mprof[base + cc] : 0.0;
+          kv[d] = kprof[base + cc];
         }
+        double sacc = 0.0;
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+          const int c = i + (d - 3) * g.ns;
+          if ((c >= 0) && (c < n)) {
+            sacc = fma(mv[d], X[c], sacc);
+            kprof[base + c] = fma(-dmu, mv[d], kv[d]);
+          }
+        }
+        Z1[i] = g.muhat[i];
+        Z2[i] = sacc;
       }
       mu_app = mu;
     }
-    for (int i = lane; i < n; i += 64) Z1[i] = g.muhat[i];
-    mass_matvec(n, g.ns, mprof, g.first_col, g.prof_off, X, Z2, lane);
     __syncthreads();
     factor_forward<W, R>(g, kprof, SC, LEV, Z1, Z2, PIV, STG, urows, WP, lane);
     __syncthreads();
