@@ -95,14 +95,27 @@ __device__ __forceinline__ double fast_rcp(double a) {
 }
 
 // y = K x over the flat profile: entries are lane-strided (coalesced loads, no
-// dependence between iterations) and accumulated into y[row] with LDS atomics.
-// y must be zeroed and synchronised by the caller.
+// dependence between iterations) and accumulated into y[row] with LDS atomics.  Groups of
+// U entries per lane issue all their loads before the first atomic, so a group waits on
+// memory once.  y must be zeroed and synchronised by the caller.
 __device__ void flat_matvec(int P, const double* __restrict__ kp, const unsigned* __restrict__ rc,
                             const double* __restrict__ x, double* y, int lane) {
-  for (int e = lane; e < P; e += 64) {
+  constexpr int U = 8;
+  int e0 = lane;
+  for (; e0 + 64 * (U - 1) < P; e0 += 64 * U) {
+    unsigned v[U];
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v[u] = rc[e0 + 64 * u];
+      a[u] = kp[e0 + 64 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) atomicAdd(&y[v[u] >> 16], a[u] * x[v[u] & 0xffffu]);
+  }
+  for (int e = e0; e < P; e += 64) {
     const unsigned v = rc[e];
-    const int row = (int)(v >> 16), col = (int)(v & 0xffffu);
-    atomicAdd(&y[row], kp[e] * x[col]);
+    atomicAdd(&y[v >> 16], kp[e] * x[v & 0xffffu]);
   }
 }
 
