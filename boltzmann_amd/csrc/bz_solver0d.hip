@@ -403,11 +403,15 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
 // positions j mod W (chunk bases are multiples of W).
 template <int W>
 __device__ void back_substitute(int n, const double* __restrict__ urows, int WP, double* __restrict__ Y1,
-                                double* __restrict__ Y2, double* __restrict__ stg, int lane) {
+                                double* __restrict__ Y2, double* __restrict__ stg_, int lane) {
   constexpr int CH = 2 * W;
   constexpr int PER = (CH * ((W + 1) & ~1) + 63) / 64;   // doubles per lane per chunk
   const int csize = CH * WP;
-  double* Y = lane == 0 ? Y1 : Y2;
+  // Y1, Y2 and stg are in LDS: say so, so that the accesses are LDS instructions (counted
+  // apart from the global loads of the next chunk) rather than generic ones
+  typedef __attribute__((address_space(3))) double lds_double;
+  lds_double* Y = (lds_double*)(lane == 0 ? Y1 : Y2);
+  lds_double* stg = (lds_double*)stg_;
   double z[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) z[k] = 0.0;
@@ -444,7 +448,7 @@ __device__ void back_substitute(int n, const double* __restrict__ urows, int WP,
       for (int tt = CH - 1; tt >= 0; --tt) {
         const int j = j0 + tt;
         if (j < n) {
-          const double* ur = stg + tt * WP;
+          const lds_double* ur = stg + tt * WP;
           double sum = Y[j];
 #pragma unroll
           for (int k = W - 1; k >= 1; --k) sum = fma(-ur[k], z[(tt + k) % W], sum);
