@@ -214,6 +214,13 @@ __device__ void mass_matvec(int n, int ns, const double* __restrict__ mprof, con
   }
 }
 
+// Pointers into LDS, marked as such: a function the compiler keeps out of line would
+// otherwise access LDS with generic (flat) instructions, which also wait on global loads.
+typedef __attribute__((address_space(3))) double lds_double;
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) short lds_short;
+typedef __attribute__((address_space(1))) double glb_double;
+
 // Register state of the factorisation: R levels of row slots per lane.  Each slot also
 // holds the row that takes it next (nrow), so that one LDS read of that row's entry gives
 // all a slot change needs.  b1/b2 hold the right-hand-side corrections -sum_j m_rj y_j
@@ -252,11 +259,11 @@ __device__ __forceinline__ void row_enter(int2 v, int row, int& fc, int& off, in
 // in-order memory counter, i.e. on every column prefetch issued before it).
 template <int W, int R, int RA, bool SL, bool TAIL>
 __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<W, R>& s,
-                                             const double* __restrict__ kprof, const int2* __restrict__ s_succ,
-                                             double* __restrict__ Y1, double* __restrict__ Y2,
-                                             double* __restrict__ s_piv, double* __restrict__ s_junk,
-                                             double* __restrict__ urows, int WP, int lane,
-                                             double* __restrict__ lout) {
+                                             const glb_double* __restrict__ kprof, const lds_int* __restrict__ s_succ,
+                                             lds_double* __restrict__ Y1, lds_double* __restrict__ Y2,
+                                             lds_double* __restrict__ s_piv, lds_double* __restrict__ s_junk,
+                                             glb_double* __restrict__ urows, int WP, int lane,
+                                             glb_double* __restrict__ lout) {
 #pragma unroll
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
@@ -276,7 +283,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
     // branch): level rr of the pivot lane into s_piv, the others into their own row of
     // s_junk.
     {
-      double* dst = (lane == lp) ? s_piv : s_junk + lane;
+      lds_double* dst = (lane == lp) ? s_piv : s_junk + lane;
 #pragma unroll
       for (int r = 0; r < RA; ++r) {
         if (rr == r) {
@@ -327,18 +334,19 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
           s.vmask[r] = 0u;
           s.b1[r] = 0.0;
           s.b2[r] = 0.0;
-          row_enter(s_succ[nr >= 0 ? nr : 0], nr, s.fc[r], s.off[r], s.nrow[r]);
+          const int q = nr >= 0 ? nr : 0;
+          row_enter(make_int2(s_succ[2 * q], s_succ[2 * q + 1]), nr, s.fc[r], s.off[r], s.nrow[r]);
         }
       }
     }
     // --- branch-free prefetch of column j+2W (invalid columns read entry 0 and are masked);
-    // a 32-bit byte offset from the instance's profile base
+    // a 32-bit index from the instance's profile base
 #pragma unroll
     for (int r = 0; r < RA; ++r) {
       const int c = j + 2 * W;
       const bool valid = (c >= s.fc[r]) && (c <= min(s.row[r] + bw, n - 1));
-      const unsigned boff = valid ? ((unsigned)(s.off[r] + c) << 3) : 0u;
-      s.nk[r][t] = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(kprof) + boff);
+      const unsigned idx = valid ? (unsigned)(s.off[r] + c) : 0u;
+      s.nk[r][t] = kprof[idx];
       s.vmask[r] = (s.vmask[r] & ~(1u << t)) | ((valid ? 1u : 0u) << t);
     }
   }
@@ -352,19 +360,29 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
 // first) go to urows.  The column prefetch is branch-free so that every step issues
 // a fixed number of loads and the memory-counter waits can be counted statically.
 template <int W, int R, bool SL = false>
-__device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof,
-                               const int2* __restrict__ s_succ, const short* __restrict__ s_lev,
-                               double* __restrict__ Y1, double* __restrict__ Y2, double* __restrict__ s_piv,
-                               double* __restrict__ s_junk, double* __restrict__ urows, int WP, int lane,
-                               double* __restrict__ lout = nullptr) {
+__device__ void factor_forward(const GridDev& g, const double* __restrict__ kprof_,
+                               const int2* __restrict__ succ_, const short* __restrict__ lev_,
+                               double* __restrict__ y1_, double* __restrict__ y2_, double* __restrict__ piv_,
+                               double* __restrict__ junk_, double* __restrict__ urows_, int WP, int lane,
+                               double* __restrict__ lout_ = nullptr) {
   const int n = g.n, bw = g.bw;
+  const glb_double* kprof = (const glb_double*)kprof_;
+  glb_double* urows = (glb_double*)urows_;
+  glb_double* lout = (glb_double*)lout_;
+  const lds_int* s_succ = (const lds_int*)succ_;          // the row table as int pairs
+  const lds_short* s_lev = (const lds_short*)lev_;
+  lds_double* Y1 = (lds_double*)y1_;
+  lds_double* Y2 = (lds_double*)y2_;
+  lds_double* s_piv = (lds_double*)piv_;
+  lds_double* s_junk = (lds_double*)junk_;
   FactorState<W, R> s;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int rw = g.slot_first[r * 64 + lane];
     s.row[r] = rw;
     const int lc = (rw >= 0) ? min(rw + bw, n - 1) : -1;
-    row_enter(s_succ[rw >= 0 ? rw : 0], rw, s.fc[r], s.off[r], s.nrow[r]);
+    const int q = rw >= 0 ? rw : 0;
+    row_enter(make_int2(s_succ[2 * q], s_succ[2 * q + 1]), rw, s.fc[r], s.off[r], s.nrow[r]);
     s.b1[r] = 0.0;
     s.b2[r] = 0.0;
     s.vmask[r] = 0u;
