@@ -223,8 +223,8 @@ typedef __attribute__((address_space(1))) double glb_double;
 
 // Register state of the factorisation: R levels of row slots per lane.  Each slot also
 // holds the row that takes it next (nrow), so that one LDS read of that row's entry gives
-// all a slot change needs.  b1/b2 hold the right-hand-side corrections -sum_j m_rj y_j
-// accumulated since the row entered; its entry value is added at its pivot step.
+// all a slot change needs.  b1/b2 hold the row's two right-hand-side values, eliminated up
+// to the current step (read from Y1/Y2 when the row enters).
 template <int W, int R>
 struct FactorState {
   int row[R], fc[R], off[R], nrow[R];
@@ -268,7 +268,6 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
   for (int t = 0; t < W; ++t) {
     const int j = j0 + t;
     if (TAIL && j >= n) break;
-    const double e1 = Y1[j], e2 = Y2[j];     // entry values of row j's right-hand sides
     // --- locate the pivot row j: lane lp, level rr (both wave-uniform)
     int lp = 0, rr = 0;
 #pragma unroll
@@ -300,7 +299,7 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
     double pv[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) pv[k] = s_piv[k];
-    const double pb1 = e1 + s_piv[W], pb2 = e2 + s_piv[W + 1];
+    const double pb1 = s_piv[W], pb2 = s_piv[W + 1];
     const double rinv = fast_rcp(pv[0]);
     // --- U row j (1/U_jj first) and the eliminated rhs; every lane stores entry lane % WP
     {
@@ -332,9 +331,9 @@ __device__ __forceinline__ void factor_chunk(int j0, int n, int bw, FactorState<
           const int nr = s.nrow[r];
           s.row[r] = nr;
           s.vmask[r] = 0u;
-          s.b1[r] = 0.0;
-          s.b2[r] = 0.0;
           const int q = nr >= 0 ? nr : 0;
+          s.b1[r] = (nr >= 0) ? Y1[q] : 0.0;     // entry values (M uhat, M x) of the new row
+          s.b2[r] = (nr >= 0) ? Y2[q] : 0.0;
           row_enter(make_int2(s_succ[2 * q], s_succ[2 * q + 1]), nr, s.fc[r], s.off[r], s.nrow[r]);
         }
       }
@@ -383,8 +382,8 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
     const int lc = (rw >= 0) ? min(rw + bw, n - 1) : -1;
     const int q = rw >= 0 ? rw : 0;
     row_enter(make_int2(s_succ[2 * q], s_succ[2 * q + 1]), rw, s.fc[r], s.off[r], s.nrow[r]);
-    s.b1[r] = 0.0;
-    s.b2[r] = 0.0;
+    s.b1[r] = (rw >= 0) ? Y1[q] : 0.0;
+    s.b2[r] = (rw >= 0) ? Y2[q] : 0.0;
     s.vmask[r] = 0u;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
@@ -402,8 +401,8 @@ __device__ void factor_forward(const GridDev& g, const double* __restrict__ kpro
   // entry, the compiler's wait analysis would put a full vector-memory wait in every chunk.
   // (gfx9 s_waitcnt fields: vmcnt 0, expcnt 7, lgkmcnt 15.)
   __builtin_amdgcn_s_waitcnt(0x0F70);
-  // Y1/Y2 hold the entry values of the rhs (M uhat, M x) until pivot step i overwrites
-  // Y*[i] with the eliminated value; step j reads Y*[j] before it writes it.
+  // Y1/Y2 double as the entry values of the rhs (M uhat, M x): a row entering its
+  // slot during step j has index > j, and Y*[i] is only overwritten at pivot step i.
   const int nfull = n / W;
   for (int ch = 0; ch < nfull; ++ch)
     factor_chunk<W, R, R, SL, false>(ch * W, n, bw, s, kprof, s_succ, Y1, Y2, s_piv, s_junk, urows, WP, lane,
