@@ -946,6 +946,18 @@ __global__ __launch_bounds__(64) void k_ee_precond_apply(GridDev g, BatchDev b, 
     }
     const double* S = SB + st * LBUF;
     const int base = PO[r0];
+    if (PO[r1] - base > 128 * (r1 - r0)) {
+      // long rows (more than 128 entries on average, few rows per stage): each row's sum over
+      // its columns left of the diagonal is split across the 64 lanes and reduced
+      for (int i = r0; i < r1; ++i) {
+        const int fc = FC[i], off = PO[i] - base - fc;
+        double acc = 0.0;
+        for (int j = fc + lane; j < i; j += 64) acc = fma(-S[off + j], Y2[j], acc);
+        acc = wave_sum(acc);
+        if (lane == 0) Y2[i] += acc;
+        __syncthreads();
+      }
+    } else
     // blocks of 64 rows, lane t owning row blk0 + t: first the columns left of the block
     // (values final), then the unit lower triangle of the block, one column per step with
     // the finished value broadcast from its lane
