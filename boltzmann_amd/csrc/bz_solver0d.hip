@@ -531,7 +531,7 @@ __host__ __device__ __forceinline__ int zpad(int n, int npr) { return (4 * npr >
 // Two waves per SIMD (register budget 256 incl. spills) measured faster than one
 // wave holding every value in registers.
 template <int W, int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_newton0d(GridDev g, BatchDev b) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W >= 17 ? 1 : 2))) void k_newton0d(GridDev g, BatchDev b) {
   extern __shared__ double lds[];
   const int n = g.n, lane = threadIdx.x, inst = blockIdx.x;
   constexpr int WP = (W + 1) & ~1;
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(64) void k_ee_apply(GridDev g, BatchDev b, int mode
 // z1 = (K_x - mu M)^-1 M uhat with s1 = uhat.z1 (permuted order); then every application
 // is a mass product, one profile forward solve and one back substitution.
 template <int W, int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_ee_factor(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W >= 17 ? 1 : 2))) void k_ee_factor(
     GridDev g, BatchDev b, const int* __restrict__ active, double* __restrict__ lprof, double* __restrict__ z1,
     double* __restrict__ s1) {
   extern __shared__ double lds[];
@@ -1056,7 +1056,7 @@ int launch_newton(const GridDev& g, const BatchDev& b, hipStream_t stream, const
 // test follow the reference loop (the convergence test of an iterate is made before
 // its update, and the update is still applied), per instance.
 template <int W, int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_transient0d(GridDev g, BatchDev b,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W >= 17 ? 1 : 2))) void k_transient0d(GridDev g, BatchDev b,
                                                                                            double cdt,
                                                                                            const double* ustart,
                                                                                            double* avg) {
