@@ -188,14 +188,32 @@ class VStep:
         return mv
 
     def _pc(self, E):
+        """Block preconditioner: each x point multiplies the inverse of its E bin's block
+        (bz_nearest_bin picks the bin).  The bins are fixed for the whole v-step, so the points
+        are grouped by bin once here; every application is then one batched FP64 GEMM over
+        the bins in use (each block read once), between a gather and a scatter of the points."""
         nx = E.shape[0]
         bins = torch.empty(nx, dtype=torch.int32, device=self.dev)
         _lib.call("bz_nearest_bin", E, nx, self.Evals, self.Evals.shape[0], bins)
         dof = self.dof
+        b64 = bins.long()
+        order = torch.argsort(b64, stable=True)
+        uniq, counts = torch.unique_consecutive(b64[order], return_counts=True)
+        nb = int(uniq.numel())
+        maxc = int(counts.max())
+        start = torch.cumsum(counts, 0) - counts
+        group = torch.repeat_interleave(torch.arange(nb, device=self.dev), counts, output_size=nx)
+        pos = torch.arange(nx, device=self.dev) - start[group]
+        idx = torch.zeros((nb, maxc), dtype=torch.long, device=self.dev)   # padding reads point 0
+        idx[group, pos] = order
+        Pg = self.PmatE[uniq].contiguous()                                  # (nb, dof, dof)
 
         def mv(x):
+            X = x.reshape(dof, nx)
+            U = X[:, idx].permute(1, 0, 2).contiguous()                     # (nb, dof, maxc)
+            Yg = torch.bmm(Pg, U)                                           # (nb, dof, maxc)
             y = torch.empty((dof, nx), dtype=torch.float64, device=self.dev)
-            _lib.call("bz_binned_block_apply", self.PmatE, dof, bins, x.reshape(dof, nx).contiguous(), nx, y)
+            y[:, order] = Yg[group, :, pos].t()
             return y.reshape(-1)
         return mv
 
