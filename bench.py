@@ -9,10 +9,12 @@ excitation (11.5 eV) / ionization (15.76 eV) cross sections; n0 = 3.22e22 m^-3,
 Tg = 300 K, no e-e.
 
 One step = one batched solve of all 4096 instances from the Maxwellian initial
-guess to |r| <= 1e-12 |r0| (exact-Jacobian Newton, at most 12 iterations) plus
+guess to |r| <= 1e-14 |r0| (exact-Jacobian Newton, at most 12 iterations) plus
 normalisation and QoIs, with operators and instance data resident in HBM.  Only
-converged instances count in solves/s: status 0, or a stop with |r| / |r0| < 1e-11
-(the FP64 roundoff floor of the residual).
+converged instances count in solves/s: status 0, i.e. the true residual (recomputed in
+compensated arithmetic on every iteration) passed |r| <= 1e-14 |r0|.  A seeded sample of
+256 instances is checked against the FP64 oracle (rate floor 1e-12); the line reports
+how many of them meet 1e-8 on every QoI.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
 """
@@ -33,11 +35,13 @@ XSEC = os.path.join(ROOT, "boltzmann_amd", "data", "eAr_synthetic_3proc.lxcat")
 B = 4096
 NR, LMAX, TE, EVMAX, Q = 128, 2, 4.0, 80.0, 5
 SEED = 20251202
-RTOL = 1e-12
+RTOL = 1e-14          # the compensated residual's floor is ~1e-16 |r0|; 1e-12 |r0| leaves ~1e-10 in
+                      # the QoIs of ill-conditioned instances
 RTOL_EE = 1e-15        # e-e solves run to the FP64 floor of |F| (status 0, or 3: stopped at the floor)
 MAX_ITER = 12
-RATE_FLOOR = 1e-9
-FLOOR_REL = 1e-11      # a stop with |r| / |r0| below this is at the FP64 roundoff floor
+RATE_FLOOR = 1e-12     # rates below RATE_FLOOR * max_g k are compared absolutely (|dk| <= 1e-8 max_g k)
+ACC_SAMPLE = 256
+QOI_TOL = 1e-8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_newton0d_b4096_n387_hbm_traffic.txt")
 
 
@@ -181,9 +185,13 @@ def plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0, rtol, max_iter):
     return conv / dt, float(torch.cat(its).double().mean()), conv, dt
 
 
-def accuracy_check(h_gpu, q_gpu, idx):
-    """FP64 oracle (exact-Jacobian Newton to roundoff) on a seeded sample; max relative
-    error per QoI with the tiny-rate floor, and EDF relative L2."""
+def _oracle_chunk(idx):
+    """Oracle references (exact-Jacobian Newton to 1e-15) for instances idx of the batch."""
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except ImportError:
+        pass
     from oracle import operators as op, solver0d as s0, xsec
     recs, _ = xsec.read_cross_section_data(XSEC)
     procs = op.processes_from_lxcat(recs)
@@ -191,18 +199,53 @@ def accuracy_check(h_gpu, q_gpu, idx):
     ops = s0.BatchedOps(g, procs)
     prm = instances(B)
     sub = {k: v[..., idx] for k, v in prm.items()}
-    h_ref, q_ref, _ = s0.steady_state_solve_each(ops, sub, ops.initial(len(idx)), 0.0, 1e-15, 30, jacobian="exact")
+    h_ref, q_ref, infos = s0.steady_state_solve_each(ops, sub, ops.initial(len(idx)), 0.0, 1e-15, 30,
+                                                     jacobian="exact")
+    ok = np.array([(not d["diverged"]) and d["rel"][0] < 1e-10 for d in infos])
+    return h_ref, q_ref, ok
+
+
+def start_oracle_pool(idx, workers=None):
+    """Start the oracle references of the accuracy sample in a pool of worker processes.
+    Call before anything initialises the GPU: the workers are forked and use numpy only."""
+    import multiprocessing as mp
+    workers = workers or max(1, min(16, len(os.sched_getaffinity(0)) - 1))
+    chunks = [c for c in np.array_split(np.asarray(idx), workers) if len(c)]
+    pool = mp.get_context("fork").Pool(len(chunks))
+    return pool, pool.map_async(_oracle_chunk, chunks)
+
+
+def accuracy_from_refs(h_gpu, q_gpu, status, idx, refs):
+    """Per-instance max relative error over every QoI (rate floor RATE_FLOOR) and EDF L2 of
+    the GPU results against the oracle references; counts the instances at <= 1e-8."""
+    h_ref = np.concatenate([r[0] for r in refs], axis=1)
+    q_ref = {k: np.concatenate([r[1][k] for r in refs], axis=-1) for k in refs[0][1]}
+    ok_ref = np.concatenate([r[2] for r in refs])
     errs = {}
+    per = np.zeros(len(idx))
     for k in ("energy", "mobility", "diffusion"):
-        errs[k] = float(np.max(np.abs(q_gpu[k][idx] - q_ref[k]) / np.abs(q_ref[k])))
+        e = np.abs(q_gpu[k][idx] - q_ref[k]) / np.abs(q_ref[k])
+        errs[k] = e
+        per = np.maximum(per, e)
     rb = q_ref["rates"]
     ra = q_gpu["rates"][:, idx]
     kmax = np.max(np.abs(rb), axis=0, keepdims=True)
     denom = np.where(np.abs(rb) < RATE_FLOOR * kmax, kmax, np.abs(rb))
-    errs["rates"] = float(np.max(np.abs(ra - rb) / denom))
+    er = np.max(np.abs(ra - rb) / denom, axis=0)
+    errs["rates"] = er
+    per = np.maximum(per, er)
     hg = h_gpu[:, idx]
-    errs["edf_l2"] = float(np.max(np.linalg.norm(hg - h_ref, axis=0) / np.linalg.norm(h_ref, axis=0)))
-    return errs
+    el2 = np.linalg.norm(hg - h_ref, axis=0) / np.linalg.norm(h_ref, axis=0)
+    errs["edf_l2"] = el2
+    per = np.maximum(per, el2)
+    counted = (status[idx] == 0) & ok_ref
+    out = {k: float(np.max(v[counted])) if counted.any() else None for k, v in errs.items()}
+    out.update({"sample": int(len(idx)), "counted_status0": int(counted.sum()),
+                "within_1e-8": int(((per <= QOI_TOL) & counted).sum()),
+                "status0_but_oracle_unconverged": int(((status[idx] == 0) & ~ok_ref).sum()),
+                "rate_floor": RATE_FLOOR,
+                "worst_instances": [int(i) for i in np.asarray(idx)[np.argsort(-np.where(counted, per, 0))[:3]]]})
+    return out
 
 
 def _grid_and_solver(Nr=NR, lmax=LMAX, q=Q, ee=False):
@@ -422,6 +465,10 @@ def main():
         {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow, "serve": bench_serve}[args.workload](args)
         return
 
+    acc_idx = np.sort(np.random.default_rng(SEED + 1).choice(B, ACC_SAMPLE, replace=False))
+    pool = refs_async = None
+    if not args.no_check:
+        pool, refs_async = start_oracle_pool(acc_idx)
     import torch
     from boltzmann_amd.cross_section import xsections_from_file
     from boltzmann_amd.solver0d import Batched0D
@@ -480,11 +527,11 @@ def main():
     st = solver.tables.st
     fl_it = algorithmic_flops_per_iteration(st, g.n)
     flops_launch = float(fl_it * iters.sum())
+    traffic = measured_traffic()
     achieved = flops_launch / (kernel_ms.mean() * 1e-3) / 1e12
     peak = 78.6
     dense_ref = (g.n - 1) ** 3 + 2 * (g.n - 1) ** 2      # reference model of the dense Jacobian solve
-    rel = (sc["res_abs"] / sc["res0"]).cpu().numpy()
-    conv = int(((status == 0) | (rel < FLOOR_REL)).sum())
+    conv = int((status == 0).sum())
     value = conv * args.steps / elapsed
 
     line = {
@@ -501,20 +548,22 @@ def main():
         "newton_iters": {"mean": float(iters.mean()), "max": int(iters.max()),
                          "converged": int((status == 0).sum()), "of": B},
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "hbm_frac": (traffic / (kernel_ms.mean() * 1e-3) / 8e12) if traffic else None,
                      "kernel": "k_prepare0d + k_newton0d<11,2> (one bz_steady_solve_0d call)",
                      "kernel_ms": float(kernel_ms.mean()),
-                     "note": "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
-                             "achieved = algorithmic flops of the profile-LU Newton iterations run / launch time"},
+                     "note": "FP64 VALU kernel limited by the issue rate of the sequential pivot steps (no MFMA); "
+                             "peak = MI355X FP64 vector peak 78.6 TFLOP/s; achieved = algorithmic flops of the "
+                             "profile-LU Newton iterations run / launch time; hbm_frac = traffic / launch time / 8 TB/s"},
         "reference_equivalent_tflops": float(dense_ref * iters.sum() / (kernel_ms.mean() * 1e-3) / 1e12),
     }
     if not args.no_check:
-        idx = np.random.default_rng(SEED + 1).choice(B, 16, replace=False)
         qn = {k: v.cpu().numpy() for k, v in q.items()}
-        line["accuracy"] = accuracy_check(h.cpu().numpy(), qn, idx)
-        line["accuracy"]["sample"] = 16
-        line["accuracy"]["rate_floor"] = RATE_FLOOR
+        refs = refs_async.get()
+        pool.close()
+        pool.join()
+        line["accuracy"] = accuracy_from_refs(h.cpu().numpy(), qn, status, acc_idx, refs)
     if not args.no_plain:
         pv, pit, pconv, pdt = plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0.clone(), RTOL, MAX_ITER)
         line["plain_fp64_gpu"] = {"value": pv, "unit": "solves/s", "mean_iters": pit, "converged": pconv,

@@ -36,7 +36,8 @@ class Batch0D(ctypes.Structure):
                 ("x", c_void_p), ("kprof", c_void_p), ("urows", c_void_p), ("vec", c_void_p), ("iters", c_void_p),
                 ("status", c_void_p), ("res_abs", c_void_p), ("res0", c_void_p), ("atol", c_double),
                 ("rtol", c_double), ("alpha_min", c_double), ("alpha_rho", c_double), ("ls_c", c_double),
-                ("max_iter", c_int), ("mode", c_int), ("tg_scale", c_double), ("tol_abs", c_void_p)]
+                ("max_iter", c_int), ("mode", c_int), ("tg_scale", c_double), ("tol_abs", c_void_p),
+                ("lprof", c_void_p)]
 
 
 class EE0D(ctypes.Structure):

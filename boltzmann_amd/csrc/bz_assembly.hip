@@ -93,9 +93,12 @@ __global__ void k_collision_radial(const double* __restrict__ t, int npr, const 
       const double x3 = x * x * x;
       const double wt = w * vth * x3 * sig;
       if (ptype == 0) {
+        // B_p(v') - B_p(v) with v - v' = v (2m/M) / (1 + sqrt(1 - 2m/M)), free of cancellation
+        const double dv = x * (2.0 * mByM) / (1.0 + vscale);
+        const double db = bspline3_diff_down(t, npr, p, x, dv);
         double dterm = 0.0;
         if (kappa != 0.0 && rp >= 0 && rp <= 3) dterm = kappa * w * x3 * sig * D[rp] * D[rk];
-        a0 += wt * bk * (bpp - bp) - dterm;
+        a0 += wt * bk * db - dterm;
       } else {
         a0 += wt * bk * (split * bpp - bp);
       }

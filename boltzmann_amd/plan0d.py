@@ -22,6 +22,7 @@ import torch
 from . import _lib
 
 PREFETCH_WINDOWS = 2   # rows enter their slot 2W steps before their first column
+VEC_ROWS = 12          # n-vectors of per-instance work space (bz_batch0d_t.vec)
 
 
 def permutation(npr, ns):
@@ -117,11 +118,13 @@ def plan_structure(mask_orig, npr, ns, window_fn):
 class SolverTables:
     """Device tables of one grid for bz_steady_solve_0d."""
 
-    def __init__(self, vgrid, ops, growth_mask, u_orig, mass_radial, extra_mask=None):
+    def __init__(self, vgrid, ops, growth_mask, u_orig, mass_radial, extra_mask=None, growth_ops=None):
         """vgrid: VGrid; ops: list of dense (n,n) device tensors [C_0..C_{S-1}, C_Tg, A_E, M];
         growth_mask[m] (m < nop-1): operator m contributes to the growth rate mu = w.x;
         u_orig: mass vector u (original order, device); mass_radial: radial mass block;
-        extra_mask: (n, n) bool pattern of per-instance operators added later (e-e)."""
+        extra_mask: (n, n) bool pattern of per-instance operators added later (e-e);
+        growth_ops: operators (parallel to ops[:-1]) whose W_m = G_m^T M^-1 u form the growth
+        vectors (default: ops themselves)."""
         dev = vgrid.dev
         self.dev = dev
         npr, ns, n = vgrid.npr, vgrid.num_sh, vgrid.n
@@ -179,7 +182,8 @@ class SolverTables:
         tmp = torch.empty(n, dtype=torch.float64, device=dev)
         for m in range(self.nop - 1):
             if growth_mask[m]:
-                _lib.call("bz_gemv", ops[m].contiguous(), n, n, 1, v, 1.0, tmp)   # Op^T v
+                G = ops[m] if growth_ops is None else growth_ops[m]
+                _lib.call("bz_gemv", G.contiguous(), n, n, 1, v, 1.0, tmp)   # Op^T v
                 _lib.call("bz_permute", tmp, n, ns, npr, 0, self.wsp[m])
         self.W = st["W"]
         self.WP = (self.W + 1) & ~1
@@ -193,4 +197,4 @@ class SolverTables:
             max_row=int(np.diff(st["prof_off"]).max()))
 
     def scratch_bytes(self, B):
-        return B * (self.P + self.n * self.WP + 4 * self.n) * 8
+        return B * (self.P + self.n * self.WP + VEC_ROWS * self.n) * 8

@@ -258,6 +258,14 @@ class ServingEngine:
                     time.sleep(poll)
                 elif inflight:
                     torch.cuda.synchronize()
+                else:
+                    # offline and idle: move the simulated clock to the next event, the next
+                    # arrival or the end of the wait of the oldest e-e queue still growing
+                    events = [reqs[i].arrival] if i < len(reqs) else []
+                    events += [qq[0].arrival + self.ee_max_wait for qk, qq in queues.items() if qq and qk[3]]
+                    if not events:
+                        raise RuntimeError("serving loop: queued requests but nothing can be admitted")
+                    sim_now = max(sim_now, min(events))
         makespan = time.perf_counter() - t0
         lat = np.array([results[r.rid].latency for r in reqs])
         ok = np.array([results[r.rid].status in (0, 3) for r in reqs])

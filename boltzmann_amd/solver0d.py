@@ -13,8 +13,12 @@ import numpy as np
 import torch
 
 from . import _lib
-from .plan0d import SolverTables
+from .plan0d import VEC_ROWS, SolverTables
 from .vspace import C_GAMMA, QE, ME, TEMP_EV_FAC, VGrid, XSection
+
+
+# ptype of the processes that change the number of electrons (ionisation 2, attachment 3)
+NONCONSERVATIVE = (2, 3)
 
 
 class Batched0D:
@@ -33,9 +37,13 @@ class Batched0D:
         n = g.n
         dev = g.dev
         C = [torch.zeros((n, n), dtype=torch.float64, device=dev) for _ in species]
+        C_grow = [torch.zeros((n, n), dtype=torch.float64, device=dev) for _ in species]
         Cg = torch.zeros((n, n), dtype=torch.float64, device=dev)
         for x in self.xsecs:
-            C[species.index(x.species)] += g.collision_matrix(x, tgK=0.0)
+            Cx = g.collision_matrix(x, tgK=0.0)
+            C[species.index(x.species)] += Cx
+            if x.ptype in NONCONSERVATIVE:
+                C_grow[species.index(x.species)] += Cx
             if x.ptype == 0:
                 Cg += g.gas_temperature_matrix(x)
         self.elastic_species = [species.index(x.species) for x in self.xsecs if x.ptype == 0]
@@ -51,14 +59,19 @@ class Batched0D:
         self.rates = torch.stack([g.rate_op(x) for x in self.xsecs]).contiguous()
         self.mw0 = 1.0 / (math.sqrt(math.pi) * g.vth) ** 3
         self.u = (self.mass * (self.mw0 * g.vth ** 3)).contiguous()
-        gm = [True] * self.S + ([False, False] if growth == "collisions" else [True, True])
+        # growth vectors W_m = Op_m^T M^-1 u from the processes that change the electron number
+        # (ionisation, attachment) and, on the single path, the field term; for particle-
+        # conserving operators (elastic, excitation, gas temperature) the vector is zero, which
+        # the reference's product only reproduces up to roundoff
+        gm = [True] * self.S + ([False, False] if growth == "collisions" else [False, True])
         self.ee_pattern = ee_pattern
         extra = None
         if ee_pattern:
             # e-e couples (p, q) to (k, l) for |k - p| <= 3 and every pair of harmonics
             pr = torch.arange(n, device=dev) // g.num_sh
             extra = (pr[:, None] - pr[None, :]).abs() <= 3
-        self.tables = SolverTables(g, C + [Cg, A, M], gm, self.u, self.mass_radial, extra_mask=extra)
+        self.tables = SolverTables(g, C + [Cg, A, M], gm, self.u, self.mass_radial, extra_mask=extra,
+                                   growth_ops=C_grow + [Cg, A])
         self.h0 = g.maxwellian_init()
         self._scratch = {}
 
@@ -72,7 +85,7 @@ class Batched0D:
         return dict(
             kprof=torch.empty((B, t.P), dtype=torch.float64, device=dev),
             urows=torch.empty((B, t.n, t.WP), dtype=torch.float64, device=dev),
-            vec=torch.empty((B, 4, t.n), dtype=torch.float64, device=dev),
+            vec=torch.empty((B, VEC_ROWS, t.n), dtype=torch.float64, device=dev),
             iters=torch.empty(B, dtype=torch.int32, device=dev),
             status=torch.empty(B, dtype=torch.int32, device=dev),
             res_abs=torch.empty(B, dtype=torch.float64, device=dev),
@@ -125,10 +138,12 @@ class Batched0D:
         return sc
 
     def transient_step(self, x, n0, Tg, E, ns_by_n0, dt_tau, atol=1e-10, rtol=1e-10, max_iter=30, ustart=None,
-                       avg=None, tg_scale=1.0, stream=None):
+                       avg=None, tg_scale=1.0, stream=None, store_factor=True):
         """One backward-Euler step of size dt_tau for every instance (bz_transient_step_0d):
         x (n, B) holds the initial iterate on entry and the new state on exit; ustart is the
         state at the start of the step (None: x itself); avg += ustart + x_new if given.
+        store_factor: keep the L factor of the frozen matrix (B x P doubles of scratch) so
+        that it is factorised once per step instead of once per Newton iteration.
         Returns the scratch dict (iters, status, res_abs, res0)."""
         B = x.shape[1]
         assert x.shape[0] == self.g.n and x.dtype == torch.float64 and x.is_contiguous()
@@ -144,6 +159,10 @@ class Batched0D:
                           res_abs=sc["res_abs"].data_ptr(), res0=sc["res0"].data_ptr(), atol=float(atol),
                           rtol=float(rtol), alpha_min=1e-16, alpha_rho=0.5, ls_c=0.0, max_iter=int(max_iter), mode=0,
                           tg_scale=float(tg_scale))
+        if store_factor:
+            if sc.get("lprof") is None:
+                sc["lprof"] = torch.empty((B, self.tables.P), dtype=torch.float64, device=self.g.dev)
+            bt.lprof = sc["lprof"].data_ptr()
         _lib.call("bz_transient_step_0d", self.tables.struct, bt, float(dt_tau), ustart, avg, stream=stream)
         return sc
 

@@ -109,7 +109,7 @@ typedef struct {
   double* x;              /* (n, B): initial guess in, solution out (unnormalised) */
   double* kprof;          /* [B][P] scratch                                    */
   double* urows;          /* [B][n][W rounded up to even] scratch              */
-  double* vec;            /* [B][4][n] scratch                                 */
+  double* vec;            /* [B][12][n] scratch                                */
   int* iters;             /* [B]                                               */
   int* status;            /* [B] 0 converged, 1 iteration cap, 2 line search failed */
   double* res_abs;        /* [B] final |r|                                     */
@@ -119,6 +119,9 @@ typedef struct {
   int mode;               /* 0: reference Jacobian, 1: exact Jacobian          */
   double tg_scale;        /* C_Tg coefficient is n0 * Tg * tg_scale            */
   const double* tol_abs;  /* [B] per-instance absolute tolerance; NULL: atol + rtol * |r0| */
+  double* lprof;          /* [B][P] optional scratch of bz_transient_step_0d: the L factor of the
+                             frozen matrix, so that it is factorised once per step (NULL: once
+                             per Newton iteration); ignored by the steady solvers   */
 } bz_batch0d_t;
 
 /* Batched damped-Newton steady state of all B instances in one launch (one wavefront

@@ -9,15 +9,17 @@ Restates (BESolver/python/...):
     (collision_operator_spherical.py:786-971);
   * CollisionOpSP.coulomb_collision_op_assembly: cc_a, cc_b[(p,q),(k,l),(r,s)] =
     sum_x w Ia/Ib_{qls}(x) for |k - p|, |r - p| within the reference's window
-    (collision_operator_spherical.py:973-1050), with the weak-form terms from
-    boltzmann_amd/ee_terms_table.py (derived symbolically by ee_terms_gen.py);
+    (collision_operator_spherical.py:973-1050), with the weak-form terms of the reference's
+    own table (cc_terms.py Ia / Ib, read as text into tests/golden/cc_terms_ref.npz by
+    tests/golden/make_cc_terms.py; reference_terms() below);
   * the operator cc = M^-1 (cc_a . hl_op + cc_b . gl_op) (bte_0d3v_batched.py:444-476) and
     the Coulomb prefactor gamma_a (bte_0d3v_batched.py:652-661);
   * the e-e residual and Jacobian terms of get_rhs_and_jacobian
     (bte_0d3v_batched.py:702-709, 734-743).
 
-Pinned by tests/test_oracle_ee.py: Pm/Qm against exact integrals; the operator conserves
-particles, momentum and energy; a Maxwellian is in its null space (C(f_M, f_M) = 0).
+Pinned by tests/test_oracle_ee.py: the term table is the reference's (monomials and values at
+seeded points); Pm/Qm against exact integrals; the operator conserves particles, momentum
+and energy; a Maxwellian is in its null space (C(f_M, f_M) = 0).
 """
 import math
 
@@ -88,8 +90,31 @@ def rosenbluth_ops(g: op.Grid, Minv, m_ab=1.0):
     return Minv @ hl, Minv @ gl
 
 
-def cc_tensors(g: op.Grid, terms, lmax=None):
-    """Dense cc_a, cc_b (n, n, n) (small grids only)."""
+_REF_TERMS = None
+
+
+def reference_terms():
+    """The reference's weak-form term table (cc_terms.py Ia / Ib for (q, l, s) <= 4) from the
+    committed golden data: {'a'|'b': {(q, l, s): [(coef, pow v, dp, dk, dr), ...]}}."""
+    global _REF_TERMS
+    if _REF_TERMS is None:
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "tests", "golden", "cc_terms_ref.npz")
+        d = np.load(path)
+        out = {}
+        for kind in ("a", "b"):
+            tab = {}
+            for row in d[f"terms_{kind}"]:
+                key = (int(row[0]), int(row[1]), int(row[2]))
+                tab.setdefault(key, []).append((float(row[7]), int(row[3]), int(row[4]), int(row[5]), int(row[6])))
+            out[kind] = tab
+        _REF_TERMS = out
+    return _REF_TERMS
+
+
+def cc_tensors(g: op.Grid, terms=None, lmax=None):
+    """Dense cc_a, cc_b (n, n, n) (small grids only); terms: the reference's table by default."""
+    terms = reference_terms() if terms is None else terms
     ns = g.num_sh
     lmax = g.lmax if lmax is None else lmax
     p_ = g.sp_order

@@ -195,6 +195,29 @@ def processes_from_lxcat(recs):
                     r["attachment_energy"], r["species"], r["process"]) for r in recs]
 
 
+def elastic_gain_difference(g: Grid, x, c_mu):
+    """B_p(x sqrt(1 - c_mu)) - B_p(x) for every basis function p (rows) and point x (columns),
+    as minus the integral of B_p' over [x - dv, x], dv = x c_mu / (1 + sqrt(1 - c_mu)), split at
+    the knot inside (dv is far below the knot spacing), 2 Gauss-Legendre points per piece
+    (B_p' is quadratic on a piece).  Piece widths come from dv and from x minus the knot, not
+    from the rounded end point x - dv."""
+    x = np.asarray(x, dtype=np.float64)
+    dv = x * c_mu / (1.0 + math.sqrt(1.0 - c_mu))
+    t = g.t
+    kn = np.searchsorted(t, x - dv, side="right")      # first knot > x - dv
+    knot = t[np.minimum(kn, len(t) - 1)]
+    inside = (kn < len(t)) & (knot < x)
+    roff = np.where(inside, x - knot, 0.0)             # the split point is x - roff
+    out = np.zeros((g.num_p, x.size))
+    c = 1.0 / (2.0 * math.sqrt(3.0))
+    for a_off, b_off in ((dv, roff), (roff, np.zeros_like(x))):
+        h = a_off - b_off
+        m = x - 0.5 * (a_off + b_off)
+        for y in (m - c * h, m + c * h):
+            out -= 0.5 * h * g.V(y, 1)
+    return out
+
+
 def collision_radial(g: Grid, proc: Process, tgK=0.0):
     """(q0, q1) radial blocks of _LOp_eulerian_radial_only, CG branch
     (collision_operator_spherical.py:96-223).  q0 is the l = 0 block, q1 the l >= 1 block."""
@@ -207,10 +230,13 @@ def collision_radial(g: Grid, proc: Process, tgK=0.0):
         vpost = x * math.sqrt(1.0 - c_mu)
         kappa = (KB * tgK * c_mu * 0.5 / ME) / vth
         V = g.V(x)
-        Vp = g.V(vpost)
         D = g.V(x, 1)
         wt = w * vth * x ** 3 * sig
-        q0 = (Vp - V) @ (V * wt).T - kappa * (D * (w * x ** 3 * sig)) @ D.T
+        # B_p(v') - B_p(v) (collision_operator_spherical.py:136-144) evaluated as the integral of
+        # B_p' over [v', v] (2 Gauss points per knot piece, exact): the same quantity without the
+        # cancellation of two values that differ by ~m/M (which leaves ~1e-11 relative noise)
+        dV = elastic_gain_difference(g, x, c_mu)
+        q0 = dV @ (V * wt).T - kappa * (D * (w * x ** 3 * sig)) @ D.T
         q1 = -(V * wt) @ V.T
         return q0, q1
     if proc.type in ("EXCITATION", "IONIZATION", "ATTACHMENT"):

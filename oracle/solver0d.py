@@ -23,6 +23,10 @@ import numpy as np
 from . import operators as op
 
 
+# process types that change the number of electrons (they alone carry a growth term)
+NONCONSERVATIVE = ("IONIZATION", "ATTACHMENT")
+
+
 class BatchedOps:
     """Per-grid operators of the batched path (bte_0d3v_batched.py:360-442)."""
 
@@ -38,9 +42,13 @@ class BatchedOps:
         M = op.mass_matrix(g)
         Minv = op.inverse_mass_matrix(M)
         FOp = [np.zeros((n, n)) for _ in species]
+        FOp_grow = [np.zeros((n, n)) for _ in species]
         FOp_g = np.zeros((n, n))
         for p in procs:
-            FOp[species.index(p.species)] += op.collision_matrix(g, p, tgK=0.0)
+            Cp = op.collision_matrix(g, p, tgK=0.0)
+            FOp[species.index(p.species)] += Cp
+            if p.type in NONCONSERVATIVE:
+                FOp_grow[species.index(p.species)] += Cp
             if p.type == "ELASTIC":
                 FOp_g += op.gas_temperature_matrix(g, p)
         self.M = M
@@ -59,7 +67,12 @@ class BatchedOps:
         mw0 = op.maxwellian0(g.vth)
         self.u = self.mass * mw0 * g.vth ** 3
         self.Q, self.R = qr_basis(self.u, g.num_p, g.num_sh)
-        self.W = np.einsum("v,svu->su", self.u, self.c_en)
+        # growth vectors W_s = u^T M^-1 C_s (bte_0d3v_batched.py:636) from the processes that
+        # change the electron number; for elastic and excitation processes u^T M^-1 C is zero
+        # (particle conservation, checked by the oracle's tests) and the reference's product
+        # leaves only roundoff there, ~1e-17 of the ionisation part, which moves rates at
+        # 1e-11 of the largest by ~1e-6 (see test_oracle_solver.py)
+        self.W = np.einsum("v,svu->su", self.u, np.array([Minv @ F for F in FOp_grow]))
         self.ee = ee
 
     def initial(self, B):
@@ -242,13 +255,18 @@ class SingleOps:
         self.M = op.mass_matrix(g)
         self.Minv = op.inverse_mass_matrix(self.M)
         C = np.zeros((g.n, g.n))
+        C_grow = np.zeros((g.n, g.n))
         for p in procs:
             x = ns_by_n0[species.index(p.species)] * n0
             if p.type == "ATTACHMENT":
-                C += x ** 2 * op.collision_matrix(g, p, tgK=Tg)
+                Cp = x ** 2 * op.collision_matrix(g, p, tgK=Tg)
             else:
-                C += x * op.collision_matrix(g, p, tgK=Tg)
+                Cp = x * op.collision_matrix(g, p, tgK=Tg)
+            C += Cp
+            if p.type in NONCONSERVATIVE:
+                C_grow += Cp
         self.C = C
+        self.C_grow = C_grow           # the part of C that changes the electron number
         self.Araw = op.advection_matrix(g)
         self.mass = op.mass_op(g)
         self.temp = op.temp_op(g)
@@ -298,7 +316,9 @@ def single_steady_state(ops: SingleOps, E_field, h_init=None, rtol=1e-13, atol=1
     CpE = ops.Minv @ (ops.C + Emat)
     u, Q, R = ops.u, ops.Q, ops.R
     QT = Q.T
-    W = u @ CpE
+    # W = u^T M^-1 (C + E A) (bte_0d3v_solver.py:787) without the exact zeros of the
+    # particle-conserving processes (see BatchedOps)
+    W = u @ (ops.Minv @ (ops.C_grow + Emat))
 
     def res(x):
         return QT @ (CpE @ x) - (W @ x) * (QT @ x)

@@ -16,13 +16,12 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from helpers import instance_params, oracle_batched  # noqa: E402
 from oracle import ee as eeo, solver0d as s0  # noqa: E402
-from boltzmann_amd.ee_terms_table import TERMS  # noqa: E402
 
 
 def main():
     ops, g, procs = oracle_batched(128, 2)
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
-    A, Bt = eeo.cc_tensors(g, TERMS)
+    A, Bt = eeo.cc_tensors(g)
     cc = eeo.ee_operator(ops.Minv, A, Bt, hl, gl)
     B = 4
     prm = instance_params(B, seed=387)

@@ -65,9 +65,8 @@ def test_dropin_ee_with_warm_start():
     args.ee_collisions = 1
     h, q = bte.solve(0, ff, 0.0, 1e-15, 60, "steady-state")
     ops, g, procs = oracle_batched(24, 1)
-    from boltzmann_amd.ee_terms_table import TERMS
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
-    A, Bt = eeo.cc_tensors(g, TERMS)
+    A, Bt = eeo.cc_tensors(g)
     x0, _, _ = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, 1e-14, 40, jacobian="exact")
     ops.ee = {"cc": eeo.ee_operator(ops.Minv, A, Bt, hl, gl)}
     _, q_ref, _ = s0.steady_state_solve_each(ops, prm, x0, 0.0, 1e-14, 60, jacobian="exact")

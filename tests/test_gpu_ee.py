@@ -26,7 +26,6 @@ def _d(a):
 @pytest.fixture(scope="module")
 def setup():
     from boltzmann_amd.bte_ee import EEOperator
-    from boltzmann_amd.ee_terms_table import TERMS
     from boltzmann_amd.solver0d import Batched0D
     print("case: e-e assembly Nr=24 lmax=1", flush=True)
     ops, g, procs = oracle_batched(24, 1)
@@ -35,7 +34,7 @@ def setup():
     eop = EEOperator(vg, s)
     torch.cuda.synchronize()
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
-    A, Bt = eeo.cc_tensors(g, TERMS)
+    A, Bt = eeo.cc_tensors(g)
     return ops, g, s, eop, hl, gl, A, Bt
 
 
@@ -191,7 +190,6 @@ def test_newton_krylov_pieces(setup):
 def test_steady_state_with_ee_lmax4():
     """e-e at lmax 4 (window 20 of the solver, 125 harmonic triples) against the oracle."""
     from boltzmann_amd.bte_ee import EEOperator
-    from boltzmann_amd.ee_terms_table import TERMS
     from boltzmann_amd.solver0d import Batched0D
     print("case: e-e steady state Nr=12 lmax=4 B=2", flush=True)
     ops, g, procs = oracle_batched(12, 4)
@@ -203,7 +201,7 @@ def test_steady_state_with_ee_lmax4():
     prm["ne"] = prm["n0"] * np.array([1e-2, 1e-4])
     x0, _, _ = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, 1e-13, 40, jacobian="exact")
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
-    A, Bt = eeo.cc_tensors(g, TERMS)
+    A, Bt = eeo.cc_tensors(g)
     ops.ee = {"cc": eeo.ee_operator(ops.Minv, A, Bt, hl, gl)}
     try:
         _, q_ref, _ = s0.steady_state_solve_each(ops, prm, x0, 0.0, 1e-14, 60, jacobian="exact")
@@ -238,7 +236,6 @@ def test_steady_state_with_ee_n387():
     """
     import os
     from boltzmann_amd.bte_ee import EEOperator
-    from boltzmann_amd.ee_terms_table import TERMS
     from boltzmann_amd.solver0d import Batched0D
     print("case: e-e steady state Nr=128 lmax=2 B=4 vs stored oracle", flush=True)
     ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ee_n387.npz"))
@@ -248,7 +245,7 @@ def test_steady_state_with_ee_n387():
     eop = EEOperator(vg, s)
     ns_, npr = g.num_sh, g.num_p
     hl, gl = eeo.rosenbluth_ops(g, ops.Minv)
-    A, Bt = eeo.cc_tensors(g, TERMS)
+    A, Bt = eeo.cc_tensors(g)
     # assembly parity
     for l in range(ns_):
         for dev_op, r_ in ((eop.H, hl), (eop.G, gl)):

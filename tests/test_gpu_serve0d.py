@@ -109,3 +109,19 @@ def test_realtime_arrivals(engine):
     off, _ = engine.run([dataclasses.replace(r, arrival=0.0) for r in reqs], realtime=False)
     for a, b in zip(res, off):
         _same(a, b, exact=True)
+
+
+def test_offline_staggered_arrivals_with_ee(engine):
+    """Offline (simulated clock) with staggered arrivals: an e-e request waits for its batch to
+    grow while later requests have not arrived yet and nothing is in flight.  The simulated
+    clock must move on to the next event (arrival or the end of the e-e wait), so the run
+    ends and serves every request."""
+    print("case: offline serving, staggered arrivals, e-e request first", flush=True)
+    reqs = _requests(6, seed=13, ee_every=3)
+    arr = [0.0, 0.01, 0.02, 0.5, 0.51, 0.9]
+    reqs = [dataclasses.replace(r, arrival=a) for r, a in zip(reqs, arr)]
+    assert reqs[0].ion_deg > 0
+    res, st = engine.run(reqs, realtime=False)
+    print("offline staggered stats", st, flush=True)
+    assert len(res) == len(reqs)
+    assert st["converged"] >= len(reqs) - 1

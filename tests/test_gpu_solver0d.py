@@ -28,11 +28,9 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
 
 
-def _converged(info, floor=1e-12):
-    """Residual at the FP64 roundoff floor: status 0, or a stop (line search or the
-    verification of the residual from scratch) with |r| / |r0| below the floor."""
-    rel = info["res_abs"] / info["res0"]
-    return (info["status"] == 0) | (rel < floor)
+def _converged(info):
+    """Status 0: the true residual (recomputed on every iteration) passed the test."""
+    return info["status"] == 0
 
 
 def _gpu_solve(solver, prm, mode, rtol=1e-13, max_iter=60):
@@ -71,18 +69,18 @@ def test_exact_newton_matches_oracle_small(small):
 
 
 def test_reference_newton_matches_oracle_small(small):
-    """mode 0 reproduces the reference iteration, including its line-search stalls at low
-    E/N (bte_0d3v_batched.py:947-956): same outcome and residual as the oracle loop."""
+    """mode 0 runs the reference iteration (bte_0d3v_batched.py:882-979: Picard Jacobian, Armijo
+    halving on the true residual): every instance the oracle loop converges is converged and
+    matches; an instance where the oracle loop stalls (the line search fails, at low E/N) is
+    not reported converged.  Which of the two stall first there (line-search failure or the
+    iteration cap) depends on roundoff in the stalled steps and is not compared."""
     ops, solver, prm, h_ref, q_ref = small
     h, q, info = _gpu_solve(solver, prm, mode=0, rtol=1e-13, max_iter=400)
     _, q_o, infos = s0.steady_state_solve_each(ops, prm, ops.initial(12), 0.0, 1e-13, 400)
     for i in range(12):
-        diverged = infos[i]["diverged"]
-        assert (info["status"][i] == 2) == diverged, (i, info["status"][i], infos[i])
-        if diverged:
-            assert abs(info["res_abs"][i] - infos[i]["abs"][0]) / infos[i]["abs"][0] < 1e-6
-        else:
-            assert info["status"][i] == 0
+        converged = (not infos[i]["diverged"]) and infos[i]["rel"][0] <= 1e-13
+        assert (info["status"][i] == 0) == converged, (i, info["status"][i], infos[i])
+        if converged:
             e = qoi_errors({k: (v[..., i:i + 1]) for k, v in q.items()}, {k: v[..., i:i + 1] for k, v in q_ref.items()})
             assert max(e.values()) < QOI_TOL, (i, e)
 
@@ -125,15 +123,38 @@ def test_near_floor_instance_of_headline_batch():
     eliminated the right-hand sides as corrections added to the entry value at the pivot
     raised that floor to 1.6e-12 |r0|: the instance then missed rtol 1e-12 and one rate moved
     by 3e-8. Solved alone (each instance's result is the same alone and in the batch) to
-    rtol 1e-12 within 12 iterations, it must converge and match the oracle's exact Newton."""
+    rtol 1e-14 within 12 iterations, it must converge and match the oracle's exact Newton."""
     print("case: solver0d Nr=128 lmax=2, instance 385 of the headline batch", flush=True)
     ops, g, procs, solver = _setup(128, 2)
     full = instance_params(4096)
     assert abs(full["EbyN"][385] - 30.770052458834954) < 1e-9
     prm = {k: v[..., 385:386] for k, v in full.items()}
-    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-12, max_iter=12)
+    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-14, max_iter=12)
     assert info["status"][0] == 0, info
     h_ref, q_ref, _ = s0.steady_state_solve_each(ops, prm, ops.initial(1), 0.0, 1e-15, 30, jacobian="exact")
     assert rel_err(h, h_ref) < QOI_TOL
     errs = qoi_errors(q, q_ref)
+    assert max(errs.values()) < QOI_TOL, errs
+
+
+@pytest.mark.parametrize("idx", [3252, 2401, 3797, 730])
+def test_tail_rate_instances_of_headline_batch(idx):
+    """Instances of the 4096-instance headline batch whose ionisation rate (4e-6 to 6e-5 of
+    the largest rate, or 6e-12 for instance 2401 at 1.1 Td) moved by 1.2e-8 to 5.2e-8 when the
+    Newton update was formed directly from the two bordered solves: the fixed point then
+    carried the roundoff of the unpivoted factorisation.  In residual form (the third
+    right-hand side carries the compensated residual) each must converge to rtol 1e-14
+    within 12 iterations and match the oracle's exact Newton to 1e-8 on every QoI, with the
+    rate floor 1e-12."""
+    print(f"case: solver0d Nr=128 lmax=2, instance {idx} of the headline batch", flush=True)
+    ops, g, procs, solver = _setup(128, 2)
+    full = instance_params(4096)
+    prm = {k: v[..., idx:idx + 1] for k, v in full.items()}
+    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-14, max_iter=12)
+    assert info["status"][0] == 0, info
+    h_ref, q_ref, _ = s0.steady_state_solve_each(ops, prm, ops.initial(1), 0.0, 1e-15, 30, jacobian="exact")
+    assert rel_err(h, h_ref) < QOI_TOL
+    errs = qoi_errors(q, q_ref)
+    print(f"instance {idx}: iters {info['iters'][0]} |r|/|r0| {info['res_abs'][0] / info['res0'][0]:.2e} {errs}",
+          flush=True)
     assert max(errs.values()) < QOI_TOL, errs

@@ -79,3 +79,26 @@ def test_time_harmonic_cycles_match_reference(setup):
         qa["rates"] = qg["rates"][:, i:i + 1]
         errs = qoi_errors(qa, qr)
         assert max(errs.values()) < 1e-9, errs
+
+
+def test_stored_factor_equals_refactorisation(setup):
+    """The step with the stored L factor (factorised once, forward solves with the stored
+    multipliers afterwards) gives bitwise the iterates of refactorising the frozen matrix in
+    every iteration: same multipliers, same elimination order per row."""
+    ops, g, s = setup
+    print("case: transient step Nr=24 lmax=1 B=6, stored factor vs refactorisation", flush=True)
+    B = 6
+    prm = instance_params(B, seed=23)
+    u = ops.initial(B)
+    u = _d(u / (ops.u @ u))
+    out = []
+    for store in (False, True):
+        x = u.clone()
+        sc = s.transient_step(x, _d(prm["n0"]), _d(prm["Tg"]), _d(prm["E"]), _d(prm["ns_by_n0"]), 1e-10, 0.0,
+                              1e-12, 60, ustart=u, store_factor=store)
+        torch.cuda.synchronize()
+        out.append((x.clone(), sc["iters"].clone(), sc["status"].clone()))
+    assert torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][2], out[1][2])
+    assert torch.equal(out[0][0], out[1][0])
+    assert int(out[1][1].max()) > 1

@@ -11,6 +11,7 @@ term table (boltzmann_amd/ee_terms_table.py):
   (decreasing with resolution), and vanishes on a Maxwellian to second order.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -20,6 +21,61 @@ from helpers import oracle_procs
 from oracle import bspline, ee
 from oracle import operators as op
 from oracle import solver0d as s0
+
+
+def _eval_terms(terms, pts):
+    """Value of sum c v^a B_p^(dp) B_k^(dk) B_r^(dr) at the golden points."""
+    val = np.zeros(pts["vr"].shape)
+    for c, pw, dp, dk, dr in terms:
+        f = c * pts["vr"] ** pw
+        for a, d in (("p", dp), ("k", dk), ("r", dr)):
+            f = f * pts[["B_%s_vr", "DB_%s_dvr", "DB_%s_dvr_dvr"][d] % a]
+        val = val + f
+    return val
+
+
+def test_table_matches_reference_cc_terms():
+    """The product's term table against the reference's own Ia / Ib (cc_terms.py, read as
+    text into tests/golden/cc_terms_ref.npz): every (q, l, s) <= 4 of the reference is in the
+    table and no other; the integrands agree at 16 seeded points to 1e-13; the monomials
+    (power of v and derivative orders, after combining like terms) and their coefficients
+    agree term by term.  The oracle's e-e tensors are built from the reference's table."""
+    from boltzmann_amd.ee_terms_table import TERMS
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cc_terms_ref.npz"))
+    pts = {str(k): v for k, v in zip(d["points_names"], d["points"])}
+    ref = ee.reference_terms()
+    for kind in ("a", "b"):
+        keys = [tuple(int(v) for v in k) for k in d[f"keys_{kind}"]]
+        mine = {k for k, v in TERMS[kind].items() if v}
+        assert set(keys) == mine, (kind, sorted(set(keys) ^ mine))
+        for k, vals in zip(keys, d[f"values_{kind}"]):
+            got = _eval_terms(TERMS[kind][k], pts)
+            assert np.max(np.abs(got - vals)) <= 1e-13 * np.max(np.abs(vals)), (kind, k)
+
+            def combine(ts):
+                out = {}
+                for c, *m in ts:
+                    out[tuple(m)] = out.get(tuple(m), 0.0) + c
+                return {m: c for m, c in out.items() if abs(c) > 1e-15}
+            a, b = combine(TERMS[kind][k]), combine(ref[kind][k])
+            assert set(a) == set(b), (kind, k, a, b)
+            for m in a:
+                assert a[m] == pytest.approx(b[m], rel=1e-13), (kind, k, m)
+
+
+def test_reference_table_catches_a_changed_coefficient():
+    """The comparison above is not vacuous: one coefficient changed by 1e-9 fails it."""
+    from boltzmann_amd.ee_terms_table import TERMS
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cc_terms_ref.npz"))
+    pts = {str(k): v for k, v in zip(d["points_names"], d["points"])}
+    keys = [tuple(int(v) for v in k) for k in d["keys_b"]]
+    i = keys.index((2, 1, 3))
+    terms = list(TERMS["b"][(2, 1, 3)])
+    c, *m = terms[0]
+    terms[0] = (c * (1 + 1e-9), *m)
+    got = _eval_terms(terms, pts)
+    vals = d["values_b"][i]
+    assert np.max(np.abs(got - vals)) > 1e-13 * np.max(np.abs(vals))
 
 
 def test_table_matches_derivation_and_textbook_terms():
@@ -50,9 +106,8 @@ def grids():
     for Nr in (16, 32):
         g = op.grid_batched(Nr, 1, 4.0, 80.0, 5, [p.threshold for p in procs])
         ops = s0.BatchedOps(g, procs)
-        from boltzmann_amd.ee_terms_table import TERMS
         hl, gl = ee.rosenbluth_ops(g, ops.Minv)
-        A, Bt = ee.cc_tensors(g, TERMS)
+        A, Bt = ee.cc_tensors(g)
         out[Nr] = (g, ops, hl, gl, A, Bt)
     return out
 

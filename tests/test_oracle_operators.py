@@ -167,3 +167,50 @@ def test_maxwellian_qoi_closed_forms():
     mean_speed = g.vth * 2.0 / math.sqrt(math.pi)
     assert q["rates"][1, 0] == pytest.approx(sig0 * mean_speed, rel=1e-9)
     assert q["diffusion"][0] == pytest.approx(mean_speed / (3.0 * 2.0 * sig0), rel=1e-9)
+
+
+def _bspline_ld(t, k, x):
+    """B_k(x) in long double by the Cox-de Boor recursion on the local knots (half-open
+    intervals), for the brute-force check below."""
+    LD = np.longdouble
+    tk = np.asarray(t[k:k + 5], dtype=LD)
+    x = LD(x)
+    N = [LD(1) if tk[j] <= x < tk[j + 1] else LD(0) for j in range(4)]
+    for d in range(1, 4):
+        Nn = []
+        for j in range(4 - d):
+            a, b = tk[j + d] - tk[j], tk[j + d + 1] - tk[j + 1]
+            v = LD(0)
+            if a != 0:
+                v += (x - tk[j]) / a * N[j]
+            if b != 0:
+                v += (tk[j + d + 1] - x) / b * N[j + 1]
+            Nn.append(v)
+        N = Nn
+    return N[0]
+
+
+def test_elastic_gain_difference_without_cancellation():
+    """B_p(v sqrt(1 - 2m/M)) - B_p(v), evaluated as the integral of B_p' over [v', v], against a
+    long-double evaluation of the two values (relative 1e-12 of the difference, where the
+    FP64 difference of the values keeps only ~1e-11), and against the FP64 difference of the
+    values to the level of that cancellation."""
+    ops, g, procs = oracle_batched(32, 1)
+    c_mu = 2.0 * 1.36e-5
+    x, _ = g.quad(g.q)
+    d = op.elastic_gain_difference(g, x, c_mu)
+    naive = g.V(x * math.sqrt(1.0 - c_mu)) - g.V(x)
+    scale = np.abs(g.V(x)).max()
+    assert np.abs(d - naive).max() <= 1e-9 * scale
+    rng = np.random.default_rng(4)
+    LD = np.longdouble
+    s = np.sqrt(LD(1) - LD(c_mu))
+    for j in rng.choice(len(x), 12, replace=False):
+        xs = LD(x[j])
+        for p in range(g.num_p):
+            if not (g.t[p] <= x[j] <= g.t[p + 4]):
+                continue
+            ref = _bspline_ld(g.t, p, xs * s) - _bspline_ld(g.t, p, xs)
+            if ref == 0:
+                continue
+            assert abs(LD(d[p, j]) - ref) <= LD(1e-12) * abs(ref) + LD(1e-30), (p, j)

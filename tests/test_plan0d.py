@@ -156,3 +156,45 @@ def test_plan_config2_fits_two_rows_per_lane():
     assert st["bw"] == 10 and st["W"] == 11
     assert st["R"] == 2 and st["active_max"] <= 128
     assert st["chunk_levels"].min() >= 1 and st["chunk_levels"].max() <= st["R"]
+
+
+@pytest.mark.parametrize("jac", ["reference", "exact"])
+def test_residual_form_iteration(small, jac):
+    """Dense restatement of the kernel's residual-form step (k_newton0d): y3 = c' M uhat - K' x,
+    F = c' uhat - M^-1 y3, delta = K'^-1 y3, p = delta + a z1 + d z2 with uhat.p = 0 and
+    w.p = d (d = 0 for the reference Jacobian).  Over several damped iterations the projected
+    residual equals the reference's and p equals the reference's Q-basis Newton step."""
+    ops, g, prm = small
+    uh = ops.u / np.linalg.norm(ops.u)
+    M = ops.M
+    for i in range(4):
+        K, w = dense_ops(ops, prm, i)
+        pi = {k: (v[..., i:i + 1] if isinstance(v, np.ndarray) else v) for k, v in prm.items()}
+        res_i, jac_i = s0.batched_res_jac(ops, pi, jac)
+        x = ops.initial(1)[:, 0]
+        x = x / (ops.u @ x)
+        cp = 0.0
+        for it in range(5):
+            mu = w @ x
+            Kp = K - mu * M
+            y3 = cp * (M @ uh) - Kp @ x
+            T = np.linalg.solve(M, y3)
+            PF = -(T - (uh @ T) * uh)
+            r_ref = res_i(x[:, None])[:, 0]
+            assert abs(np.linalg.norm(PF) - np.linalg.norm(r_ref)) <= 1e-9 * np.linalg.norm(r_ref)
+            z1 = np.linalg.solve(Kp, M @ uh)
+            z2 = np.linalg.solve(Kp, M @ x)
+            delta = np.linalg.solve(Kp, y3)
+            s11, s12, s13 = uh @ z1, uh @ z2, uh @ delta
+            wz1, wz2, wz3 = w @ z1, w @ z2, w @ delta
+            if jac == "reference":
+                a, d = -s13 / s11, 0.0
+            else:
+                S = np.array([[s11, s12], [wz1, wz2 - 1.0]])
+                a, d = np.linalg.solve(S, [-s13, -wz3])
+            p = delta + a * z1 + d * z2
+            p_ref = ops.Q @ np.linalg.solve(jac_i(x[:, None])[0], -r_ref)
+            assert np.linalg.norm(p - p_ref) <= 1e-8 * np.linalg.norm(p_ref)
+            x = x + 0.7 * p           # a damped step keeps the iteration away from the root
+            x = x / (ops.u @ x)
+            cp = cp + a
