@@ -7,19 +7,19 @@ class profile_t:
         self.name = name
         self.seconds = 0.0
         self.snap = 0.0
-        self._pri_time = 0.0
+        self._t_start = 0.0
         self.iter = 0
 
     def start(self):
-        self._pri_time = perf_counter()
+        self._t_start = perf_counter()
 
     def stop(self):
-        self.snap = perf_counter() - self._pri_time
+        self.snap = perf_counter() - self._t_start
         self.seconds += self.snap
         self.iter += 1
 
     def reset(self):
         self.seconds = 0.0
         self.snap = 0.0
-        self._pri_time = 0.0
+        self._t_start = 0.0
         self.iter = 0

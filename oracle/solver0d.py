@@ -5,6 +5,8 @@ Restates (BESolver/python/...):
   * get_rhs_and_jacobian (no e-e branch)      bte_0d3v_batched.py:602-646, 763-821
   * steady_state_solve (Armijo Newton)        bte_0d3v_batched.py:882-979
   * normalized_distribution / compute_QoIs    bte_0d3v_batched.py:1163-1217
+  * refine_fixed_point: the same fixed point polished with a long-double
+    residual (a parity target, not a reference algorithm)
   * single path _QRQt, newton_solve,
     steady_state_solver, compute_QoIs        bte_0d3v_solver.py:404-460, 24-73, 741-859, 1234-1276
 
@@ -15,7 +17,9 @@ Pinned by tests/test_oracle_solver.py: the single-path fixed point equals the
 generalized eigenvector of (C + E A, M) with u^T h = 1 (scipy.linalg.eig);
 the batched fixed point satisfies the same eigen relation up to the
 reference's boundary-flux term (W excludes advection); E = 0 elastic-only
-gives the Maxwellian at Tg; Maxwellian closed forms pin the QoI vectors.
+gives the Maxwellian at Tg; Maxwellian closed forms pin the QoI vectors;
+refine_fixed_point's result has a long-double residual at the rounding of x and
+matches the FP64 Newton where that is well conditioned.
 """
 import math
 import numpy as np
@@ -192,6 +196,55 @@ def steady_state_solve(ops: BatchedOps, params, f0, atol, rtol, max_iter,
     if return_info:
         return h, qoi, {"iters": it, "abs": final_abs, "rel": final_abs / r0, "diverged": diverged}
     return h, qoi
+
+
+def refine_fixed_point(ops: BatchedOps, params, h0, iters=4):
+    """The fixed point of the batched residual (get_rhs_and_jacobian, bte_0d3v_batched.py:
+    763-821) polished to long-double accuracy, for use as a parity target.
+
+    Q^T M^-1 (K x - mu(x) M x) = 0 with x = f1 + Q y holds iff K x - mu M x = gamma M u for
+    some gamma, with u.x = 1 (the columns of Q span u-perp and M^-1 M u = u).  So this runs
+    Newton on the bordered system [K - mu M - M x w^T, -M u; u^T, 0] [dx; dgamma] =
+    -[K x - mu M x - gamma M u; u.x - 1], the residual formed in long double from the same
+    FP64 operators, the step solved in FP64.  The premultiplied Q-basis residual of the
+    reference rounds at ~1e-16 |M^-1| |K| |x|; this one at ~1e-19 |K| |x|, which matters for
+    rates 1e-11 below the largest (measured: 3e-8 relative at E/N ~ 1 Td).
+    h0: (n, B) unnormalised iterates near the fixed point.  Returns (h, qoi) as
+    steady_state_solve does."""
+    LD = np.longdouble
+    g = ops.g
+    n, B = h0.shape
+    sc = (1.0 / g.vth) * (op.QE / op.ME)
+    u = ops.u
+    M = ops.M
+    ML, uL = M.astype(LD), u.astype(LD)
+    MuL = ML @ uL
+    Mu = M @ u
+    out = np.empty_like(h0)
+    for i in range(B):
+        n0, Tg, E = params["n0"][i], params["Tg"][i], params["E"][i]
+        ns = n0 * params["ns_by_n0"][:, i]
+        KL = sum(LD(ns[s]) * ops.FOp[s].astype(LD) for s in range(len(ops.FOp)))
+        KL = KL + LD(n0 * Tg) * ops.FOp_g.astype(LD) + LD(E * sc) * ops.adv_raw.astype(LD)
+        K = KL.astype(np.float64)
+        wL = (ns[:, None] * ops.W).sum(axis=0).astype(LD)
+        w = wL.astype(np.float64)
+        x = h0[:, i].astype(LD) / (uL @ h0[:, i].astype(LD))
+        gam = LD(0)
+        for _ in range(iters):
+            xf = x.astype(np.float64)
+            mu = wL @ x
+            F = KL @ x - mu * (ML @ x) - gam * MuL
+            J = np.zeros((n + 1, n + 1))
+            J[:n, :n] = K - float(mu) * M - np.outer(M @ xf, w)
+            J[:n, n] = -Mu
+            J[n, :n] = u
+            d = np.linalg.solve(J, -np.concatenate([F.astype(np.float64), [float(uL @ x - 1)]]))
+            x = x + d[:n].astype(LD)
+            gam = gam + LD(d[n])
+        out[:, i] = x.astype(np.float64)
+    h = normalized_distribution(ops, out)
+    return h, compute_qois(ops, params, h)
 
 
 def steady_state_solve_each(ops: BatchedOps, params, f0, atol, rtol, max_iter, **kw):

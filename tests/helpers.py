@@ -30,6 +30,17 @@ def instance_params(B, seed=20251202, S=3):
     return dict(n0=n0, Tg=np.full(B, 300.0), E=EbyN * n0 * 1e-21, ns_by_n0=fac, EbyN=EbyN)
 
 
+def oracle_fixed_point(ops, prm, rtol=1e-15, max_iter=30):
+    """The oracle's exact-Jacobian Newton (the reference's loop) from the Maxwellian, then its
+    fixed point polished with a long-double residual (refine_fixed_point): the parity target
+    for tail rates, where the FP64 Newton's own floor is ~3e-8 relative at 1e-11 of the
+    largest rate."""
+    B = prm["E"].shape[0]
+    h, q, infos = s0.steady_state_solve_each(ops, prm, ops.initial(B), 0.0, rtol, max_iter, jacobian="exact")
+    h, q = s0.refine_fixed_point(ops, prm, h)
+    return h, q, infos
+
+
 def product_grid(g_oracle):
     """VGrid with exactly the oracle's knot vector."""
     from boltzmann_amd.vspace import VGrid

@@ -87,6 +87,40 @@ def test_batched_fixed_point_relation_both_jacobians():
         assert np.linalg.norm(step) <= 1e-10 * np.linalg.norm(xn[:, i])
 
 
+def test_refined_fixed_point():
+    """refine_fixed_point: (a) its long-double residual K x - mu M x - gamma M u is at the
+    rounding of x (no FP64 cancellation floor); (b) the bulk QoIs agree with the FP64 Newton
+    to its own floor, so the polish converges to the same fixed point (a dropped or
+    mis-signed term would move them by orders of magnitude more); (c) polishing again
+    changes nothing.  Instance 2 is at 1.1 Td, where the ionisation rate is ~6e-12 of the
+    elastic one and the FP64 Newton's floor shows."""
+    ops, g, procs = oracle_batched(32, 2)
+    prm = instance_params(3, seed=5)
+    prm = {k: v.copy() for k, v in prm.items()}
+    prm["E"][2] = 1.1 * prm["n0"][2] * 1e-21
+    x, q, info = s0.steady_state_solve_each(ops, prm, ops.initial(3), 0.0, 1e-15, 40, jacobian="exact")
+    h, qr = s0.refine_fixed_point(ops, prm, x)
+    LD = np.longdouble
+    sc = (1.0 / g.vth) * (op.QE / op.ME)
+    for i in range(3):
+        n0, Tg, E = prm["n0"][i], prm["Tg"][i], prm["E"][i]
+        ns = n0 * prm["ns_by_n0"][:, i]
+        KL = sum(LD(ns[s]) * ops.FOp[s].astype(LD) for s in range(len(ops.FOp))) \
+            + LD(n0 * Tg) * ops.FOp_g.astype(LD) + LD(E * sc) * ops.adv_raw.astype(LD)
+        xi = h[:, i].astype(LD) / (ops.u.astype(LD) @ h[:, i].astype(LD))
+        wl = (ns[:, None] * ops.W).sum(axis=0).astype(LD)
+        F = KL @ xi - (wl @ xi) * (ops.M.astype(LD) @ xi)
+        Mu = ops.M.astype(LD) @ ops.u.astype(LD)
+        F = F - (Mu @ F) / (Mu @ Mu) * Mu            # gamma M u is free
+        scale = np.abs(KL).max() * np.abs(xi).max()
+        assert float(np.abs(F).max() / scale) < 1e-17
+        for k in ("energy", "mobility", "diffusion"):
+            assert abs(qr[k][i] / q[k][i] - 1) < 1e-12, k
+        assert abs(qr["rates"][0, i] / q["rates"][0, i] - 1) < 1e-12
+    _, q2 = s0.refine_fixed_point(ops, prm, h, iters=2)
+    np.testing.assert_allclose(q2["rates"], qr["rates"], rtol=1e-12, atol=0)
+
+
 @pytest.mark.parametrize("Tg", [1000.0, 3000.0])
 def test_elastic_relaxes_to_gas_maxwellian_and_einstein_relation(Tg):
     el = [p for p in oracle_procs() if p.type == "ELASTIC"]
