@@ -7,7 +7,8 @@
  *   - all arithmetic is FP64; matrices are row-major;
  *   - every entry point is asynchronous on `stream` and returns 0 on success,
  *     -1 (BZ_EINVAL) for invalid sizes/arguments (nothing launched),
- *     -2 (BZ_EHIP) when a HIP launch error is reported.
+ *     -2 (BZ_EHIP) when a HIP launch error is reported,
+ *     -3 (BZ_ENOSPACE) when a request exceeds a fixed capacity of the library.
  *   - radial-major coefficient index p*ns + l (p: B-spline, l: spherical harmonic, m = 0).
  *
  * Each declaration names the reference interface it replaces
@@ -129,6 +130,15 @@ typedef struct {
  * Replaces bte_0d3v_batched.steady_state_solve + get_rhs_and_jacobian + batched_inv
  * (bte_0d3v_batched.py:602-821, 856-979). */
 int bz_steady_solve_0d(const bz_grid0d_t* grid, const bz_batch0d_t* batch, hipStream_t stream);
+
+/* bz_steady_solve_0d with the per-instance scratch (batch->kprof, urows, vec: must be NULL)
+ * taken from a static device array of the library, so that the call allocates nothing.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc).  Limit: 64 MiB per device,
+ * i.e. B * (P + n * WP + 12 n) * 8 bytes (WP = window width rounded up to even); a larger B
+ * returns -3 (BZ_ENOSPACE) with nothing launched.  Every call on a device shares that
+ * device's array: calls that overlap on one device race. */
+int bz_steady_solve_0d_noalloc(const bz_grid0d_t* grid, const bz_batch0d_t* batch, hipStream_t stream);
 
 /* One backward-Euler step of size dt_tau (= dt * tau) for all B instances from the state
  * ustart (n, B), normalised u.x = 1 (NULL: batch->x), with batch->x the initial iterate on
