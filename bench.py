@@ -67,26 +67,35 @@ def instances(nb, seed=SEED):
     return dict(n0=n0, Tg=np.full(nb, 300.0), E=EbyN * n0 * 1e-21, ns_by_n0=fac, EbyN=EbyN)
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result line is the only stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_flops_per_iteration(st, n):
-    """FP64 flops of one Newton iteration of the fused kernel for one instance:
-    profile LU without pivoting (every structurally non-zero L multiplier updates W-1
-    row entries and two right-hand sides), the two back substitutions, two profile
-    matvecs, two mass solves and the vector updates."""
+    """FP64 flops of one Newton iteration of the fused kernel for one instance, counting 2
+    per multiply-add and not the extra operations of the compensated sums: profile LU
+    without pivoting (every structurally non-zero L multiplier updates W-1 row entries and
+    three right-hand sides), the three back substitutions, one residual evaluation per
+    iteration (the line search's first trial, accepted in the common case: the profile
+    matvec, the mass band shift and the banded mass solve) and the vector updates."""
     first = st["first_col"]
     W = st["bw"] + 1
     mults = int(np.sum(np.arange(n) - first))          # envelope multipliers below the diagonal
-    f_lu = mults * (2 * (W - 1) + 2 * 2 + 1) + n * 2
-    f_bs = 2 * n * 2 * W
-    f_mv = 2 * 2 * st["P"]
-    f_ms = 2 * 4 * 2 * 3 * n
-    f_vec = 20 * n
+    f_lu = mults * (2 * (W - 1) + 2 * 3 + 1) + n * 3
+    f_bs = 3 * n * 2 * W
+    f_mv = 2 * st["P"] + 2 * 7 * n
+    f_ms = 4 * 2 * 3 * n
+    f_vec = 30 * n
     return f_lu + f_bs + f_mv + f_ms + f_vec
 
 
-def cpu_baseline(sample=32):
+def cpu_baseline(sample=256):
     """The reference CPU path restated in numpy (oracle): bte_0d3v_batched steady_state_solve
-    with xp = numpy and the reference driver defaults (atol = rtol = 1e-10, max_iter 30,
-    alpha_rho 0.5, line_search_c 1e-4), on a bounded seeded sample of the same workload."""
+    with xp = numpy (the batch-coupled Armijo Newton loop, alpha_rho 0.5, line_search_c 1e-4)
+    under the GPU leg's stopping rule: exact Jacobian, atol 0, rtol RTOL, at most 12
+    iterations, from the Maxwellian; only instances whose final |r| <= RTOL |r0| count.  A
+    bounded seeded sample of the same workload (the first `sample` instances)."""
     from oracle import operators as op, solver0d as s0, xsec
     recs, _ = xsec.read_cross_section_data(XSEC)
     procs = op.processes_from_lxcat(recs)
@@ -96,12 +105,14 @@ def cpu_baseline(sample=32):
     sub = {k: v[..., :sample] for k, v in prm.items()}
     f0 = ops.initial(sample)
     t0 = time.perf_counter()
-    s0.steady_state_solve(ops, sub, f0, 1e-10, 1e-10, 30, alpha_min=1e-16, alpha_rho=0.5, line_search_c=1e-4)
+    _, _, info = s0.steady_state_solve(ops, sub, f0, 0.0, RTOL, MAX_ITER, alpha_min=1e-16, alpha_rho=0.5,
+                                       line_search_c=1e-4, return_info=True, jacobian="exact")
     dt = time.perf_counter() - t0
+    conv = int(((info["rel"] <= RTOL) & (not info["diverged"])).sum())
     cores = len(os.sched_getaffinity(0))
-    return {"value": sample / dt, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"{sample} of the 4096 instances, reference batched loop (numpy, OpenBLAS threads = host cores), "
-                      f"{dt:.1f} s"}
+    return {"value": conv / dt, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"first {sample} of the 4096 instances, reference batched loop with the exact Jacobian to "
+                      f"|r| <= {RTOL:g} |r0| (numpy, OpenBLAS threads = host cores), {conv} converged in {dt:.1f} s"}
 
 
 def plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0, rtol, max_iter):
@@ -109,7 +120,10 @@ def plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0, rtol, max_iter):
     mixed-precision/fused targets): the Q-basis dense Newton of bte_0d3v_batched.py:
     602-821, 882-979 written with torch FP64 dense ops on the device (batched einsum for
     the Jacobian, rocSOLVER batched LU solve), with the same exact Jacobian, line search
-    and stopping rule as the fused kernel so that both take the same iterations.
+    and stopping rule as the fused kernel so that both take the same iterations.  The whole
+    batch advances together; the Armijo trials alpha = 1, 1/2, ..., 1/128 are evaluated
+    together and each instance takes the first that passes, so the loop synchronises with
+    the host once per iteration, not per trial.
     Returns (solves/s of converged instances, mean iterations, converged count, seconds)."""
     import torch
     g = solver.g
@@ -133,57 +147,61 @@ def plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0, rtol, max_iter):
     wb = (coef[:len(solver.C)].t() @ Wv)                        # (B, n)
     eye = torch.eye(n - 1, dtype=torch.float64, device=dev)
 
-    def solve_chunk(x0c, coefc, wbc):
+    NT = 8                                                    # line-search trials, alpha = 0.5^k
+
+    def residual(x, coefc, wbc):                              # x (n, Bc)
+        Kx = torch.einsum("mb,mvu,ub->vb", coefc, QTop, x)
+        mu = (wbc * x.t()).sum(1)
+        return Kx - mu[None, :] * (QT @ x), mu
+
+    def lu_solve(J, rhs):
+        # rocSOLVER's batched LU is called on slices of `chunk` systems (it reports an
+        # allocation failure for larger batches of 386^2 on this stack); the calls are
+        # asynchronous, so slicing adds no host synchronisation
+        return torch.cat([torch.linalg.solve(J[a:a + chunk], rhs[a:a + chunk]) for a in range(0, J.shape[0], chunk)])
+
+    def solve_all(x0c, coefc, wbc):
         Bc = x0c.shape[1]
-
-        def residual(x):                                      # x (n, Bc)
-            Kx = torch.einsum("mb,mvu,ub->vb", coefc, QTop, x)
-            mu = (wbc * x.t()).sum(1)
-            return Kx - mu[None, :] * (QT @ x), mu
-
         x = x0c / (u @ x0c)[None, :]
-        r, mu = residual(x)
+        r, mu = residual(x, coefc, wbc)
         rn = torch.linalg.norm(r, dim=0)
         r0 = rn.clone()
         active = torch.ones(Bc, dtype=torch.bool, device=dev)
         iters = torch.zeros(Bc, dtype=torch.int32, device=dev)
+        alphas = 0.5 ** torch.arange(NT, dtype=torch.float64, device=dev)          # (NT,)
+        coefT = coefc.repeat(1, NT)
+        wbT = wbc.repeat(NT, 1)
         for it in range(max_iter):
             active = active & (rn > rtol * r0)
-            if not bool(active.any()):
+            if not bool(active.any()):                         # one host synchronisation per iteration
                 break
             J = torch.einsum("mb,mij->bij", coefc, QTopQ) - mu[:, None, None] * eye[None]
             J = J - torch.einsum("ib,bj->bij", QT @ x, wbc @ Q)
-            dz = torch.linalg.solve(J, -r.t()).t()
+            dz = lu_solve(J, -r.t()).t()
             p = Q @ dz
-            alpha = torch.ones(Bc, dtype=torch.float64, device=dev)
-            for _ in range(60):
-                rt, _m = residual(x + alpha[None, :] * p)
-                rtn = torch.linalg.norm(rt, dim=0)
-                bad = (0.5 * rtn ** 2 > 0.5 * rn ** 2 - 1e-4 * alpha * rn ** 2) & active
-                if not bool(bad.any()):
-                    break
-                alpha = torch.where(bad, alpha * 0.5, alpha)
+            # all NT Armijo trials evaluated in one batched residual; each instance takes the
+            # first alpha that passes (the reference's halving sequence), no host synchronisation
+            xt = (x[:, None, :] + alphas[None, :, None] * p[:, None, :]).reshape(n, NT * Bc)
+            rt, _m = residual(xt, coefT, wbT)
+            rtn = torch.linalg.norm(rt, dim=0).reshape(NT, Bc)
+            ok = 0.5 * rtn ** 2 <= 0.5 * rn[None, :] ** 2 - 1e-4 * alphas[:, None] * rn[None, :] ** 2
+            first = torch.argmax(ok.to(torch.int8), dim=0)
+            alpha = torch.where(ok.any(0), alphas[first], torch.zeros_like(rn))
             step = torch.where(active, alpha, torch.zeros_like(alpha))
             x = x + step[None, :] * p
             iters += active.to(torch.int32)
-            r, mu = residual(x)
+            r, mu = residual(x, coefc, wbc)
             rn = torch.linalg.norm(r, dim=0)
         return int((rn <= rtol * r0).sum()), iters
 
-    # hipBLAS's batched LU (getrfBatched) reports an allocation failure above 64
-    # systems of 386^2 on this stack: the plain form runs the batch in chunks of 64
-    CHK = 64
+    chunk = 64
+    solve_all(x0[:, :chunk], coef[:, :chunk], wb[:chunk])          # library initialisation, untimed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    conv = 0
-    its = []
-    for a in range(0, B, CHK):
-        c, it = solve_chunk(x0[:, a:a + CHK], coef[:, a:a + CHK], wb[a:a + CHK])
-        conv += c
-        its.append(it)
+    conv, its = solve_all(x0, coef, wb)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return conv / dt, float(torch.cat(its).double().mean()), conv, dt
+    return conv / dt, float(its.double().mean()), conv, dt
 
 
 def _oracle_chunk(idx):
@@ -408,17 +426,17 @@ def bench_glow(args):
         "gmres_iterations": vs.last_iters}), flush=True)
 
 
-def bench_serve(args):
+def run_serving(N, rate):
     """Mixed 0D requests served in real time (boltzmann_amd/serve0d.py): Poisson arrivals at
-    rate --rate, each request drawing Nr in {64, 128, 256}, lmax in {1, 2, 4}, 1-3 processes,
+    `rate`, each request drawing Nr in {64, 128, 256}, lmax in {1, 2, 4}, 1-3 processes,
     E/N log-uniform over 1-1000 Td, e-e with probability 0.3 (ion_deg 10^U(-6,-2)); admission
-    by an HBM budget of 64 GiB. Reports completed solves per second and p50/p99 latency
-    (arrival to QoIs on the host)."""
+    by an HBM budget of 64 GiB.  Per-grid assembly and a 64-request warm-up run before the
+    timed run.  Returns (stats, warm-up count, setup seconds); the stats hold completed
+    solves per second and the per-request p50/p99 latency (arrival to QoIs on the host)."""
     import torch
     from boltzmann_amd.serve0d import Request, ServingEngine
     rng = np.random.default_rng(SEED)
-    N = args.requests
-    arr = np.cumsum(rng.exponential(1.0 / args.rate, N))
+    arr = np.cumsum(rng.exponential(1.0 / rate, N))
     reqs = []
     for i in range(N):
         ee = rng.uniform() < 0.3
@@ -437,9 +455,16 @@ def bench_serve(args):
     warm = [dataclasses.replace(r, rid=N + j, arrival=0.0) for j, r in enumerate(reqs[:64])]
     eng.run(warm, realtime=False)
     res, st = eng.run(reqs, realtime=True)
+    return st, len(warm), setup
+
+
+def bench_serve(args):
+    """The serving workload (run_serving) as its own bench line."""
+    N = args.requests
+    st, nwarm, setup = run_serving(N, args.rate)
     print(json.dumps({
         "metric": "0D EDF solves/s served (mixed grids, e-e p=0.3)", "value": st["solves_per_s"], "unit": "solves/s",
-        "n_gpus": 1, "steps": N, "warmup": len(warm), "ms_per_step": st["makespan_s"] / N * 1e3,
+        "n_gpus": 1, "steps": N, "warmup": nwarm, "ms_per_step": st["makespan_s"] / N * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "serving: Poisson arrivals, Nr {64,128,256} x lmax {1,2,4} x 1-3 processes, "
                                "E/N 1-1000 Td, e-e p=0.3, HBM budget 64 GiB", "requests": N, "rate_per_s": args.rate},
@@ -460,6 +485,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving latency run of the default line")
     ap.add_argument("--mode", type=int, default=1, help="1: exact Jacobian, 0: reference Jacobian")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device once (HBM byte-counter calibration)")
@@ -577,16 +603,26 @@ def main():
     }
     if not args.no_check:
         qn = {k: v.cpu().numpy() for k, v in q.items()}
+        progress("waiting for the oracle references of the accuracy sample")
         refs = refs_async.get()
         pool.close()
         pool.join()
         line["accuracy"] = accuracy_from_refs(h.cpu().numpy(), qn, status, acc_idx, refs)
     if not args.no_plain:
+        progress("plain FP64 GPU form")
         pv, pit, pconv, pdt = plain_fp64_gpu_baseline(solver, n0, Tg, E, ns, x0.clone(), RTOL, MAX_ITER)
         line["plain_fp64_gpu"] = {"value": pv, "unit": "solves/s", "mean_iters": pit, "converged": pconv,
                                   "seconds": pdt, "what": "dense Q-basis Newton, torch FP64 + rocSOLVER batched LU"}
         line["speedup_vs_plain_fp64_gpu"] = value / pv
+    if not args.no_serve:
+        progress(f"serving run, {args.requests} requests")
+        st, _, _ = run_serving(args.requests, args.rate)
+        line["serving"] = {"p50_latency_ms": st["p50_latency_s"] * 1e3, "p99_latency_ms": st["p99_latency_s"] * 1e3,
+                           "solves_per_s": st["solves_per_s"], "requests": args.requests, "converged": st["converged"],
+                           "what": f"per request, arrival to QoIs on the host: Poisson arrivals at {args.rate:g}/s, "
+                                   "mixed grids Nr {64,128,256} x lmax {1,2,4}, e-e p=0.3 (bench.py --workload serve)"}
     if not args.no_cpu:
+        progress("cpu baseline")
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 

@@ -102,6 +102,7 @@ class bte_0d3v_batched:
             self._collision_names.append(["C%d" % c for c in range(len(data))])
         self.num_collisions = len(self._coll_list[0])
         self.last_info = [None] * n_vspace_grids
+        self._qops = {}
 
     def get_collision_list(self):
         return self._coll_list
@@ -128,8 +129,11 @@ class bte_0d3v_batched:
             t = make_knots(kd, self._args.sp_order, int(self._par_nr[idx]) + 1, dg_nodes, extend_domain=False)
         elif vspace_ext == vspace_grid_extension.LINEAR:
             t = make_knots(kd, self._args.sp_order, int(self._par_nr[idx]) + 1, dg_nodes, extend_domain=True)
+        elif vspace_ext == vspace_grid_extension.LOG:
+            t = make_knots(kd, self._args.sp_order, int(self._par_nr[idx]) + 1, dg_nodes, extend_domain=False,
+                           extend_domain_with_log=True)
         else:
-            raise NotImplementedError("LOG grid extension")
+            raise NotImplementedError(vspace_ext)
         lmax = max(l for l, _ in self._par_lm[idx])
         g = VGrid(t, int(self._par_nr[idx]) + 1, lmax, self._args.spline_qpts, vth, device=self._dev)
         self._op_spec_grid[idx] = g
@@ -228,8 +232,8 @@ class bte_0d3v_batched:
                                f"or more iterations")
         return x
 
-    def solve(self, grid_idx, f0, atol, rtol, max_iter, solver_type, alpha_min=1e-16, alpha_rho=0.5,
-              line_search_c=1e-4, **kw):
+    def solve(self, grid_idx, f0, atol, rtol, max_iter, solver_type, alpha_min=1e-16, alpha_rho=0.01,
+              line_search_c=1e-3, **kw):
         if solver_type == "steady-state":
             return self.steady_state_solve(grid_idx, f0, atol, rtol, max_iter, alpha_min, alpha_rho, line_search_c,
                                            **kw)
@@ -251,6 +255,113 @@ class bte_0d3v_batched:
             return u, {"energy": q["energy"], "mobility": q["mobility"], "diffusion": q["diffusion"],
                        "rates": q["rates"].contiguous()}
         raise NotImplementedError(solver_type)
+
+    # ------------------------------------------------------------------ dense Q-basis form
+    def _qbasis_ops(self, idx):
+        """The dense operators of the reference's Q-basis residual (bte_0d3v_batched.py:
+        421-442, 602-646), on the device, built on first use: Q, R of I - p p^T with the
+        column (num_p - 1) num_sh dropped, c_en = M^-1 C_s, c_gT = M^-1 C_Tg, adv = M^-1 A,
+        their Q^T (.) and Q^T (.) Q forms and the growth vectors W_s = u^T c_en,s (from the
+        processes that change the electron number, as in the fused solver)."""
+        d = self._qops.get(idx)
+        if d is not None:
+            return d
+        s = self._solver[idx]
+        g = self._op_spec_grid[idx]
+        n = g.n
+        Minv = g.inverse_mass_matrix()
+        uh = s.u / torch.linalg.norm(s.u)
+        Impp = torch.eye(n, dtype=torch.float64, device=self._dev) - torch.outer(uh, uh)
+        Qm, Rm = torch.linalg.qr(Impp)
+        k = (g.num_p - 1) * g.num_sh
+        Q = torch.cat([Qm[:, :k], Qm[:, k + 1:]], dim=1).contiguous()
+        R = torch.cat([Rm[:k], Rm[k + 1:]], dim=0).contiguous()
+        QT = Q.t().contiguous()
+        c_en = [Minv @ C for C in s.C]
+        c_gT = Minv @ s.Cg
+        adv = Minv @ s.A
+        d = {"Q": Q, "R": R, "QT": QT,
+             "QT_Cen": torch.stack([QT @ c for c in c_en]), "QT_Cgt": QT @ c_gT, "QT_A": QT @ adv,
+             "QT_Cen_Q": torch.stack([QT @ c @ Q for c in c_en]), "QT_Cgt_Q": QT @ c_gT @ Q, "QT_A_Q": QT @ adv @ Q,
+             "W": torch.stack([s.u @ (Minv @ C) for C in s.C_grow]), "Minv": Minv,
+             "I": torch.eye(n - 1, dtype=torch.float64, device=self._dev)}
+        eop = self._op_col_ee[idx]
+        if eop is not None and getattr(self._args, "ee_collisions", 0) == 1:
+            from .bte_ee import cc_dense
+            lmap = list(range(g.num_sh))
+            hl = g.scatter_diag(eop.H, lmap)
+            gl = g.scatter_diag(eop.G, lmap)
+            cc = torch.tensordot(cc_dense(g, eop.ccA), hl, dims=([2], [0]))
+            cc += torch.tensordot(cc_dense(g, eop.ccB), gl, dims=([2], [0]))
+            d["c_ee"] = (Minv @ cc.reshape(n, -1)).reshape(n, n, n)
+            d["eop"] = eop
+        self._qops[idx] = d
+        return d
+
+    def get_rhs_and_jacobian(self, grid_idx: int, n_pts: int):
+        """bte_0d3v_batched.py:602-821: (res_func(x, time, dt), jac_func(x, time, dt)) of the
+        reference's Q-basis formulation as dense device operations: res_func maps x (n, n_pts)
+        to Q^T (M^-1 K x - mu x) (n-1, n_pts) (plus ne gamma_a Q^T (cc x) x with e-e) and
+        jac_func to the reference Jacobian (n_pts, n-1, n-1) (plus ne gamma_a Q^T (cc x +
+        cc_12 x) Q).  The parameters are read from set_boltzmann_parameter at each call, as
+        the reference does.  The fused solver does not use these: they are the reference's
+        interface for callers that run their own Newton loop."""
+        d = self._qbasis_ops(grid_idx)
+        prm = self._par_bte_params[grid_idx]
+        ee = "c_ee" in d
+
+        def get():
+            n0, Tg, E, ns_by_n0 = self._params(grid_idx, n_pts)
+            ne = _dev_tensor(prm["ne"], self._dev).reshape(-1) if ee else None
+            return n0, Tg, E, ns_by_n0, ne
+
+        def negam(x, ne):
+            out = torch.empty(x.shape[1], dtype=torch.float64, device=self._dev)
+            e = d["eop"]
+            _lib.call("bz_ee_gamma", x.contiguous(), e.n, x.shape[1], e.s.mass, e.top, ne, e.c_m0, e.c_kt, e.c_lam,
+                      e.c_gam, out)
+            return out
+
+        def res_func(x, time, dt):
+            x = _dev_tensor(x, self._dev)
+            n0, Tg, E, ns_by_n0, ne = get()
+            y = (n0 * (torch.einsum("sx,svx->vx", ns_by_n0, torch.matmul(d["QT_Cen"], x)) + Tg * (d["QT_Cgt"] @ x))
+                 + E * (d["QT_A"] @ x)
+                 - n0 * torch.einsum("sx,sx->x", ns_by_n0, d["W"] @ x) * (d["QT"] @ x))
+            if ee:
+                ccx = torch.tensordot(d["c_ee"], x, dims=([2], [0]))
+                y = y + negam(x, ne) * (d["QT"] @ torch.einsum("abc,bc->ac", ccx, x))
+            return y
+
+        def jac_func(x, time, dt):
+            x = _dev_tensor(x, self._dev)
+            n0, Tg, E, ns_by_n0, ne = get()
+            ns = n0 * ns_by_n0
+            L = torch.einsum("sx,svu->xvu", ns, d["QT_Cen_Q"]) + (n0 * Tg)[:, None, None] * d["QT_Cgt_Q"][None]
+            mu = torch.einsum("sx,sx->x", ns, d["W"] @ x)
+            L = L + E[:, None, None] * d["QT_A_Q"][None] - mu[:, None, None] * d["I"][None]
+            if ee:
+                c = torch.tensordot(d["c_ee"], x, dims=([2], [0])) + \
+                    torch.tensordot(d["c_ee"].transpose(1, 2), x, dims=([2], [0]))         # (n, n, B)
+                c = c.permute(2, 0, 1)                                                      # (B, n, n)
+                QTcQ = torch.matmul(torch.matmul(d["QT"][None], c), d["Q"][None])
+                L = L + negam(x, ne)[:, None, None] * QTcQ
+            return L
+
+        return res_func, jac_func
+
+    def batched_inv(self, grid_idx, Jmat):
+        """bte_0d3v_batched.py:856-880: the inverse of every matrix of Jmat (B, m, m).  The
+        batched LU of this stack (hipBLAS getrfBatched) refuses more than 64 systems of
+        386^2 per call, so the batch is inverted in slices of 64."""
+        pt = self.profile_tt_all[grid_idx]
+        J = _dev_tensor(Jmat, self._dev)
+        torch.cuda.synchronize()
+        pt[pp.JAC_LA_SOL].start()
+        out = torch.cat([torch.linalg.inv(J[a:a + 64]) for a in range(0, J.shape[0], 64)])
+        torch.cuda.synchronize()
+        pt[pp.JAC_LA_SOL].stop()
+        return out
 
     def normalized_distribution(self, grid_idx, ff):
         """bte_0d3v_batched.py:1204-1217."""

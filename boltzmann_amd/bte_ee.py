@@ -54,6 +54,87 @@ def _term_tables(lmax, dev):
     return out
 
 
+def rosenbluth_operators(vgrid, m_ab=1.0):
+    """H_l, G_l (ns, npr, npr): the l blocks of hl_op, gl_op of
+    CollisionOpSP.compute_rosenbluth_potentials_op (collision_operator_spherical.py:786-971),
+    from the partial moments at the grid's quadrature points (the reference's Gauss orders),
+    the potential kernels m1, m2 and the projection M^-1 [int v^2 B_p m(., k)], the last as
+    a refined banded solve."""
+    dev = vgrid.dev
+    npr, ns = vgrid.npr, vgrid.num_sh
+    # quadrature points of the grid: spline_qpts Gauss points per knot interval
+    t = vgrid.t_np
+    x, w = np.polynomial.legendre.leggauss(vgrid.q)
+    gx, gw = [], []
+    for i in range(len(t) - 1):
+        if t[i] != t[i + 1]:
+            gx.append(0.5 * (t[i + 1] - t[i]) * x + 0.5 * (t[i + 1] + t[i]))
+            gw.append(0.5 * (t[i + 1] - t[i]) * w)
+    gx = np.concatenate(gx)
+    gw = np.concatenate(gw)
+    nq = len(gx)
+    gxd = torch.from_numpy(gx).to(dev)
+    sp = vgrid.sp_order
+
+    def moment(m, kind):
+        qn = ((sp + abs(m) + 1) // 2 + 1) * (4 if kind == 0 else 32)
+        qx, qw = np.polynomial.legendre.leggauss(qn)
+        out = torch.empty((npr, nq), dtype=torch.float64, device=dev)
+        _lib.call("bz_partial_moment", vgrid.t, npr, gxd, nq, int(m), kind, torch.from_numpy(qx).to(dev),
+                  torch.from_numpy(qw).to(dev), qn, out)
+        return out
+
+    V = torch.empty((npr, nq), dtype=torch.float64, device=dev)
+    _lib.call("bz_basis_table", vgrid.t, npr, gxd, nq, 0, V)
+    Vw = (V * torch.from_numpy(gw * gx ** 2).to(dev)[None, :]).contiguous()
+    mr = vgrid.mass_radial()
+    lb = torch.empty((npr, 4), dtype=torch.float64, device=dev)
+    _lib.call("bz_band_cholesky", mr, npr, 3, lb)
+    work = torch.empty((2 * npr * npr,), dtype=torch.float64, device=dev)
+    H = torch.empty((ns, npr, npr), dtype=torch.float64, device=dev)
+    G = torch.empty((ns, npr, npr), dtype=torch.float64, device=dev)
+    raw = torch.empty((npr, npr), dtype=torch.float64, device=dev)
+    for l in range(ns):
+        m1 = torch.empty((npr, nq), dtype=torch.float64, device=dev)
+        m2 = torch.empty_like(m1)
+        _lib.call("bz_potential_kernels", l, float(m_ab), gxd, nq, npr, moment(l + 2, 0), moment(l + 4, 0),
+                  moment(l - 1, 1), moment(l - 3, 1), m1, m2)
+        for mk, dst in ((m1, H), (m2, G)):
+            # raw[p, k] = sum_x w x^2 B_p(x) m(k, x);  H_l = M_r^-1 raw as a refined solve
+            # (the explicit inverse would carry cond(M_r) roundings into H, G)
+            _lib.call("bz_gemm_strided", npr, npr, nq, 1.0, Vw, nq, 1, mk, 1, nq, raw, npr)
+            _lib.call("bz_band_chol_solve_refined", mr, lb, npr, 3, raw, npr, dst[l], work, 3)
+    return H, G
+
+
+def cc_banded(vgrid):
+    """The trilinear weak-form tensors cc_a, cc_b of coulomb_collision_op_assembly
+    (collision_operator_spherical.py:973-1050), banded: (npr, 7, 7, ns^3) each, entry
+    [p, k-p+3, r-p+3, (q ns + l) ns + s] of cc[(p,q), (k,l), (r,s)]."""
+    npr, ns = vgrid.npr, vgrid.num_sh
+    (ta_off, ta_c, ta_i), (tb_off, tb_c, tb_i) = _term_tables(vgrid.lmax, vgrid.dev)
+    glx, glw = vgrid.gl(vgrid.q)
+    nc = ns ** 3
+    ccA = torch.empty((npr, 7, 7, nc), dtype=torch.float64, device=vgrid.dev)
+    ccB = torch.empty_like(ccA)
+    _lib.call("bz_cc_banded", vgrid.t, npr, ns, glx, glw, vgrid.q, ta_off, ta_c, ta_i, tb_off, tb_c, tb_i, ccA, ccB)
+    return ccA, ccB
+
+
+def cc_dense(vgrid, band):
+    """Dense (n, n, n) tensor cc[(p,q), (k,l), (r,s)] from its banded storage (cc_banded)."""
+    npr, ns, n = vgrid.npr, vgrid.num_sh, vgrid.n
+    out = torch.zeros((npr, ns, npr, ns, npr, ns), dtype=torch.float64, device=vgrid.dev)
+    b = band.reshape(npr, 7, 7, ns, ns, ns)
+    p = torch.arange(npr, device=vgrid.dev)
+    for dk in range(-3, 4):
+        for dr in range(-3, 4):
+            sel = (p + dk >= 0) & (p + dk < npr) & (p + dr >= 0) & (p + dr < npr)
+            ps = p[sel]
+            out[ps, :, ps + dk, :, ps + dr, :] = b[ps, dk + 3, dr + 3]
+    return out.reshape(n, n, n)
+
+
 class EEOperator:
     def __init__(self, vgrid, solver, m_ab=1.0):
         """vgrid: VGrid; solver: Batched0D built with ee_pattern=True (its mass / temperature
@@ -64,52 +145,8 @@ class EEOperator:
         self.dev = dev
         npr, ns, n = vgrid.npr, vgrid.num_sh, vgrid.n
         self.npr, self.ns, self.n = npr, ns, n
-        # quadrature points of the grid: spline_qpts Gauss points per knot interval
-        t = vgrid.t_np
-        x, w = np.polynomial.legendre.leggauss(vgrid.q)
-        gx, gw = [], []
-        for i in range(len(t) - 1):
-            if t[i] != t[i + 1]:
-                gx.append(0.5 * (t[i + 1] - t[i]) * x + 0.5 * (t[i + 1] + t[i]))
-                gw.append(0.5 * (t[i + 1] - t[i]) * w)
-        gx = np.concatenate(gx)
-        gw = np.concatenate(gw)
-        nq = len(gx)
-        gxd = torch.from_numpy(gx).to(dev)
-        sp = vgrid.sp_order
-
-        def moment(m, kind):
-            qn = ((sp + abs(m) + 1) // 2 + 1) * (4 if kind == 0 else 32)
-            qx, qw = np.polynomial.legendre.leggauss(qn)
-            out = torch.empty((npr, nq), dtype=torch.float64, device=dev)
-            _lib.call("bz_partial_moment", vgrid.t, npr, gxd, nq, int(m), kind, torch.from_numpy(qx).to(dev),
-                      torch.from_numpy(qw).to(dev), qn, out)
-            return out
-
-        V = torch.empty((npr, nq), dtype=torch.float64, device=dev)
-        _lib.call("bz_basis_table", vgrid.t, npr, gxd, nq, 0, V)
-        Vw = (V * torch.from_numpy(gw * gx ** 2).to(dev)[None, :]).contiguous()
-        Minv = vgrid.mass_radial_inverse()
-        self.H = torch.empty((ns, npr, npr), dtype=torch.float64, device=dev)
-        self.G = torch.empty((ns, npr, npr), dtype=torch.float64, device=dev)
-        raw = torch.empty((npr, npr), dtype=torch.float64, device=dev)
-        for l in range(ns):
-            m1 = torch.empty((npr, nq), dtype=torch.float64, device=dev)
-            m2 = torch.empty_like(m1)
-            _lib.call("bz_potential_kernels", l, float(m_ab), gxd, nq, npr, moment(l + 2, 0), moment(l + 4, 0),
-                      moment(l - 1, 1), moment(l - 3, 1), m1, m2)
-            for mk, dst in ((m1, self.H), (m2, self.G)):
-                # raw[p, k] = sum_x w x^2 B_p(x) m(k, x);  H_l = M_r^-1 raw
-                _lib.call("bz_gemm_strided", npr, npr, nq, 1.0, Vw, nq, 1, mk, 1, nq, raw, npr)
-                _lib.call("bz_gemm_strided", npr, npr, npr, 1.0, Minv, npr, 1, raw, npr, 1, dst[l], npr)
-        (ta_off, ta_c, ta_i), (tb_off, tb_c, tb_i) = _term_tables(vgrid.lmax, dev)
-        self._tables = (ta_off, ta_c, ta_i, tb_off, tb_c, tb_i)
-        glx, glw = vgrid.gl(vgrid.q)
-        nc = ns ** 3
-        self.ccA = torch.empty((npr, 7, 7, nc), dtype=torch.float64, device=dev)
-        self.ccB = torch.empty_like(self.ccA)
-        _lib.call("bz_cc_banded", vgrid.t, npr, ns, glx, glw, vgrid.q, ta_off, ta_c, ta_i, tb_off, tb_c, tb_i,
-                  self.ccA, self.ccB)
+        self.H, self.G = rosenbluth_operators(vgrid, m_ab)
+        self.ccA, self.ccB = cc_banded(vgrid)
         # Coulomb prefactor constants (bte_0d3v_batched.py:652-661)
         vth = vgrid.vth
         mw0 = 1.0 / (math.sqrt(math.pi) * vth) ** 3

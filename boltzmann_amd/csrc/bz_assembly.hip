@@ -241,6 +241,47 @@ __global__ void k_band_chol_inverse(const double* __restrict__ lb, int npr, int 
   for (int i = 0; i < npr; ++i) inv[i * npr + c] = y[i];
 }
 
+// Solve (L L^T) y = b for one column, L banded from k_band_cholesky (hb sub-diagonals).
+__device__ void band_chol_solve_col(const double* __restrict__ lb, int npr, int hb, double* __restrict__ y) {
+  for (int i = 0; i < npr; ++i) {
+    double s = y[i];
+    for (int k = max(0, i - hb); k < i; ++k) s -= lb[i * (hb + 1) + (i - k)] * y[k];
+    y[i] = s / lb[i * (hb + 1)];
+  }
+  for (int i = npr - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k <= min(npr - 1, i + hb); ++k) s -= lb[k * (hb + 1) + (k - i)] * y[k];
+    y[i] = s / lb[i * (hb + 1)];
+  }
+}
+
+// out[:, c] = A^-1 rhs[:, c] for the banded SPD A (dense storage, |i - k| <= hb), one
+// thread per column: a Cholesky solve, then `refine` steps of iterative refinement with
+// the residual rhs - A y summed in compensated arithmetic (two_sum_acc).  The refined
+// result is the solution for the FP64 A and rhs to about one rounding, where forming
+// A^-1 explicitly and multiplying loses cond(A) roundings (the B-spline mass matrix with
+// its v^2 weight is ill-conditioned by its scaling near v = 0).
+__global__ void k_band_chol_solve_refined(const double* __restrict__ A, const double* __restrict__ lb, int npr,
+                                          int hb, const double* __restrict__ rhs, int ncol, double* __restrict__ out,
+                                          double* __restrict__ work, int refine) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  double* y = work + (long)c * 2 * npr;
+  double* r = y + npr;
+  for (int i = 0; i < npr; ++i) y[i] = rhs[(long)i * ncol + c];
+  band_chol_solve_col(lb, npr, hb, y);
+  for (int it = 0; it < refine; ++it) {
+    for (int i = 0; i < npr; ++i) {
+      double s = rhs[(long)i * ncol + c], e = 0.0;
+      for (int k = max(0, i - hb); k <= min(npr - 1, i + hb); ++k) two_sum_acc(-A[(long)i * npr + k], y[k], s, e);
+      r[i] = s + e;
+    }
+    band_chol_solve_col(lb, npr, hb, r);
+    for (int i = 0; i < npr; ++i) y[i] += r[i];
+  }
+  for (int i = 0; i < npr; ++i) out[(long)i * ncol + c] = y[i];
+}
+
 // out[k*nx + i] = B_k^(dorder)(x_i): basis Vandermonde (spec_spherical.py:338-383, Vq_r / Vdq_r).
 __global__ void k_basis_table(const double* __restrict__ t, int npr, const double* __restrict__ x, int nx,
                               int dorder, double* __restrict__ out) {
@@ -331,6 +372,15 @@ int bz_basis_table(const double* t, int npr, const double* x, int nx, int dorder
 int bz_band_cholesky(const double* A, int npr, int hb, double* lb, hipStream_t stream) {
   if (npr < 1 || hb < 0) return BZ_EINVAL;
   hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(64), 0, stream, A, npr, hb, lb);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+int bz_band_chol_solve_refined(const double* A, const double* lb, int npr, int hb, const double* rhs, int ncol,
+                               double* out, double* work, int refine, hipStream_t stream) {
+  if (npr < 1 || hb < 0 || ncol < 1 || refine < 0) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_band_chol_solve_refined, dim3(nblocks(ncol, 64)), dim3(64), 0, stream, A, lb, npr, hb, rhs,
+                     ncol, out, work, refine);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

@@ -130,6 +130,19 @@ __device__ __forceinline__ double interp_linear_flat(const double* __restrict__ 
   return slope * (x - e[lo]) + s[lo];
 }
 
+// s + c += a * b with the rounding errors of the product (fma) and of the sum (Knuth's
+// TwoSum) collected in c: the result is as accurate as a sum in twice the precision.
+__device__ __forceinline__ void two_sum_acc(double a, double b, double& s, double& c) {
+#pragma clang fp contract(off)   // a contracted s + a*b would lose the product's rounding error
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  const double t = s + p;
+  const double z = t - s;
+  const double se = (s - (t - z)) + (p - z);
+  s = t;
+  c += se + pe;
+}
+
 // 64-lane sum (result broadcast to every lane of the wave).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -51,24 +51,29 @@ __device__ void bspline3_d2(const double* __restrict__ t, int l, double x, doubl
   }
 }
 
-__device__ __forceinline__ double ipow(double x, int m) {
-  double r = 1.0;
-  const bool neg = m < 0;
-  int e = neg ? -m : m;
-  double b = x;
-  while (e) {
-    if (e & 1) r *= b;
-    b *= b;
-    e >>= 1;
+// x^m rounded once: the product carried as hi + lo (TwoProduct by fma), the reciprocal of
+// a negative power corrected by one Newton step on hi + lo
+__device__ __forceinline__ double ipow_acc(double x, int m) {
+#pragma clang fp contract(off)
+  double hi = 1.0, lo = 0.0;
+  const int e = m < 0 ? -m : m;
+  for (int k = 0; k < e; ++k) {
+    const double p = hi * x;
+    const double pe = fma(hi, x, -p) + lo * x;
+    hi = p + pe;
+    lo = pe - (hi - p);
   }
-  return neg ? 1.0 / r : r;
+  if (m >= 0) return hi + lo;
+  const double q = 1.0 / hi;
+  return q + q * (fma(-hi, q, 1.0) - lo * q);
 }
 
 // sum over the knot pieces of supp(B_i) clipped to [xb, xe] of int v^m B_i, qn Gauss points
-// per piece (integrate_xm_splines, collision_operator_spherical.py:803-817)
+// per piece (integrate_xm_splines, collision_operator_spherical.py:803-817), summed in
+// compensated arithmetic
 __device__ double piece_integral(const double* __restrict__ t, int npr, int i, int m, double xb, double xe,
                                  const double* __restrict__ qx, const double* __restrict__ qw, int qn) {
-  double s = 0.0;
+  double s = 0.0, e = 0.0;          // compensated sum over all pieces and points
   for (int kn = i + 1; kn <= i + 4; ++kn) {
     const double a = fmax(xb, t[kn - 1]), b = fmin(xe, t[kn]);
     if (b > a) {
@@ -79,11 +84,11 @@ __device__ double piece_integral(const double* __restrict__ t, int npr, int i, i
         double N[4], D[4];
         bspline3_eval(t, l, x, N, D);
         const int r = i - (l - 3);
-        s += hw * qw[g] * ipow(x, m) * N[r];
+        two_sum_acc(hw * qw[g], ipow_acc(x, m) * N[r], s, e);
       }
     }
   }
-  return s;
+  return s + e;
 }
 
 // kind 0: P[i, j] = int_{t_i}^{min(x_j, t_(i+4))} v^m B_i  (0 below the support)
@@ -116,20 +121,22 @@ __global__ void k_potential_kernels(int l, double m_ab, const double* __restrict
   const double c2 = -2.0 / (4.0 * l * l - 1.0);
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const double x = gx[e % nq];
-    m1[e] = c1 * (P2[e] / ipow(x, l + 1) + Q1[e] * ipow(x, l));
-    m2[e] = c2 * ((P2[e] / ipow(x, l - 1) - f1 * P4[e] / ipow(x, l + 1)) + (Q3[e] * ipow(x, l) - f1 * Q1[e] * ipow(x, l + 2)));
+    m1[e] = c1 * (P2[e] / ipow_acc(x, l + 1) + Q1[e] * ipow_acc(x, l));
+    m2[e] = c2 * ((P2[e] / ipow_acc(x, l - 1) - f1 * P4[e] / ipow_acc(x, l + 1)) +
+                  (Q3[e] * ipow_acc(x, l) - f1 * Q1[e] * ipow_acc(x, l + 2)));
   }
 }
 
-// C[i, j] = alpha * sum_k A[i*sai + k*sak] B[k*sbk + j*sbj]  (small set-up products)
+// C[i, j] = alpha * sum_k A[i*sai + k*sak] B[k*sbk + j*sbj]  (small set-up products, summed in
+// compensated arithmetic)
 __global__ void k_gemm_strided(int m, int n, int kk, double alpha, const double* __restrict__ A, long sai, long sak,
                                const double* __restrict__ B, long sbk, long sbj, double* __restrict__ C, long ldc) {
   const long total = (long)m * n;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e / n), j = (int)(e % n);
-    double s = 0.0;
-    for (int k = 0; k < kk; ++k) s = fma(A[i * sai + k * sak], B[k * sbk + j * sbj], s);
-    C[(long)i * ldc + j] = alpha * s;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < kk; ++k) two_sum_acc(A[i * sai + k * sak], B[k * sbk + j * sbj], s, c);
+    C[(long)i * ldc + j] = alpha * (s + c);
   }
 }
 
@@ -149,32 +156,37 @@ __global__ void k_cc_banded(const double* __restrict__ t, int npr, int ns, const
     const int k = p + dk, r = p + dr;
     double* oa = ccA + e * ncomb;
     double* ob = ccB + e * ncomb;
-    for (int c = 0; c < ncomb; ++c) { oa[c] = 0.0; ob[c] = 0.0; }
-    if (k < 0 || k >= npr || r < 0 || r >= npr) continue;
+    if (k < 0 || k >= npr || r < 0 || r >= npr) {
+      for (int c = 0; c < ncomb; ++c) { oa[c] = 0.0; ob[c] = 0.0; }
+      continue;
+    }
     // intervals l with all three supports: l in [max(p,k,r), min(p,k,r)+3]
     const int l0 = max(p, max(k, r)), l1 = min(p, min(k, r)) + 3;
-    for (int l = l0; l <= l1; ++l) {
-      if (!(t[l + 1] > t[l])) continue;
-      const double hw = 0.5 * (t[l + 1] - t[l]), cx = 0.5 * (t[l + 1] + t[l]);
-      for (int g = 0; g < q; ++g) {
-        const double x = hw * glx[g] + cx, w = hw * glw[g];
-        double N[4], D1[4], D2[4];
-        bspline3_d2(t, l, x, N, D1, D2);
-        const int rp = p - (l - 3), rk = k - (l - 3), rr = r - (l - 3);
-        const double bp[3] = {N[rp], D1[rp], D2[rp]};
-        const double bk[3] = {N[rk], D1[rk], D2[rk]};
-        const double br[3] = {N[rr], D1[rr], D2[rr]};
-        for (int c = 0; c < ncomb; ++c) {
-          double sa = 0.0;
+    for (int c = 0; c < ncomb; ++c) {
+      // every monomial of every point summed in compensated arithmetic (the terms of one
+      // combination have both signs)
+      double sa = 0.0, ea = 0.0, sb = 0.0, eb = 0.0;
+      for (int l = l0; l <= l1; ++l) {
+        if (!(t[l + 1] > t[l])) continue;
+        const double hw = 0.5 * (t[l + 1] - t[l]), cx = 0.5 * (t[l + 1] + t[l]);
+        for (int g = 0; g < q; ++g) {
+          const double x = hw * glx[g] + cx, w = hw * glw[g];
+          double N[4], D1[4], D2[4];
+          bspline3_d2(t, l, x, N, D1, D2);
+          const int rp = p - (l - 3), rk = k - (l - 3), rr = r - (l - 3);
+          const double bp[3] = {N[rp], D1[rp], D2[rp]};
+          const double bk[3] = {N[rk], D1[rk], D2[rk]};
+          const double br[3] = {N[rr], D1[rr], D2[rr]};
           for (int m = ta_off[c]; m < ta_off[c + 1]; ++m)
-            sa += ta_c[m] * ipow(x, ta_i[4 * m]) * bp[ta_i[4 * m + 1]] * bk[ta_i[4 * m + 2]] * br[ta_i[4 * m + 3]];
-          double sb = 0.0;
+            two_sum_acc(w * ta_c[m], ipow_acc(x, ta_i[4 * m]) * bp[ta_i[4 * m + 1]] * bk[ta_i[4 * m + 2]] *
+                                         br[ta_i[4 * m + 3]], sa, ea);
           for (int m = tb_off[c]; m < tb_off[c + 1]; ++m)
-            sb += tb_c[m] * ipow(x, tb_i[4 * m]) * bp[tb_i[4 * m + 1]] * bk[tb_i[4 * m + 2]] * br[tb_i[4 * m + 3]];
-          oa[c] += w * sa;
-          ob[c] += w * sb;
+            two_sum_acc(w * tb_c[m], ipow_acc(x, tb_i[4 * m]) * bp[tb_i[4 * m + 1]] * bk[tb_i[4 * m + 2]] *
+                                         br[tb_i[4 * m + 3]], sb, eb);
         }
       }
+      oa[c] = sa + ea;
+      ob[c] = sb + eb;
     }
   }
 }

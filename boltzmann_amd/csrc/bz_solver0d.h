@@ -127,19 +127,6 @@ __device__ __forceinline__ int orig_index(int i, int ns, int npr) {
   return (npr - 1 - q) * ns + l;
 }
 
-// s + c += a * b with the rounding errors of the product (fma) and of the sum (Knuth's
-// TwoSum) collected in c: the result is as accurate as a sum in twice the precision.
-__device__ __forceinline__ void two_sum_acc(double a, double b, double& s, double& c) {
-#pragma clang fp contract(off)   // a contracted s + a*b would lose the product's rounding error
-  const double p = a * b;
-  const double pe = fma(a, b, -p);
-  const double t = s + p;
-  const double z = t - s;
-  const double se = (s - (t - z)) + (p - z);
-  s = t;
-  c += se + pe;
-}
-
 __device__ __forceinline__ double fast_rcp(double a) {
   double r = __builtin_amdgcn_rcp(a);
   r = fma(r, fma(-a, r, 1.0), r);

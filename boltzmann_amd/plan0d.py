@@ -175,9 +175,12 @@ class SolverTables:
         _lib.call("bz_gemv", Mdense.contiguous(), n, n, 0, uhat_orig, 1.0, muh)
         self.muhat = torch.empty(n, dtype=torch.float64, device=dev)
         _lib.call("bz_permute", muh, n, ns, npr, 0, self.muhat)
-        minv = vgrid.inverse_mass_matrix()
+        # v = M^-1 u: the radial block solved (refined) for the ns columns of u viewed as (npr, ns)
+        lb = torch.empty((npr, 4), dtype=torch.float64, device=dev)
+        _lib.call("bz_band_cholesky", mass_radial, npr, 3, lb)
         v = torch.empty(n, dtype=torch.float64, device=dev)
-        _lib.call("bz_gemv", minv, n, n, 0, u_orig.contiguous(), 1.0, v)       # M^-1 u (M symmetric)
+        work = torch.empty(2 * n, dtype=torch.float64, device=dev)
+        _lib.call("bz_band_chol_solve_refined", mass_radial, lb, npr, 3, u_orig.contiguous(), ns, v, work, 2)
         self.wsp = torch.zeros((self.nop - 1, n), dtype=torch.float64, device=dev)
         tmp = torch.empty(n, dtype=torch.float64, device=dev)
         for m in range(self.nop - 1):
@@ -198,3 +201,31 @@ class SolverTables:
 
     def scratch_bytes(self, B):
         return B * (self.P + self.n * self.WP + VEC_ROWS * self.n) * 8
+
+
+class CPlannedGrid:
+    """The tables of one grid planned by the C entry point bz_plan_grid0d (the path of a
+    caller without this Python layer): same inputs as SolverTables, device arrays owned by
+    the library and released with bz_free_grid0d when this object goes away."""
+
+    def __init__(self, vgrid, ops, growth_ops, u_orig, mass_radial, ee_pattern=False):
+        import ctypes
+        self._keep = [o.contiguous() for o in ops] + [g.contiguous() for g in growth_ops if g is not None]
+        arr = (ctypes.c_void_p * len(ops))(*[o.data_ptr() for o in ops])
+        garr = (ctypes.c_void_p * (len(ops) - 1))(*[(g.data_ptr() if g is not None else None) for g in growth_ops])
+        self.struct = _lib.Grid0D()
+        self._handle = ctypes.c_void_p()
+        _lib.call("bz_plan_grid0d", vgrid.npr, vgrid.num_sh, len(ops), ctypes.cast(arr, ctypes.c_void_p),
+                  ctypes.cast(garr, ctypes.c_void_p), u_orig.contiguous(), mass_radial.contiguous(), int(ee_pattern),
+                  self.struct, ctypes.cast(ctypes.pointer(self._handle), ctypes.c_void_p))
+        self.P = self.struct.P
+        self.n = self.struct.n
+        W = _lib.lib().bz_solver0d_window(self.struct.bw)
+        self.W = W
+        self.WP = (W + 1) & ~1
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            _lib.lib().bz_free_grid0d(h)
+            self._handle = None

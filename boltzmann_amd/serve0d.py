@@ -92,18 +92,18 @@ class _Grid:
     def bytes_per_instance(self, ee, restart):
         t = self.solver.tables
         n = t.n
-        b = (t.P + n * t.WP + 4 * n) * 8 + 4 * n * 8 + 64 * 8
+        b = t.scratch_bytes(1) + 4 * n * 8 + 64 * 8
         if ee:
             te = self.ee()[0].tables
-            b += (te.P + n * te.WP + 4 * n) * 8                       # e-e solver scratch
+            b += te.scratch_bytes(1)                                  # e-e solver scratch
             b += ((restart + 1) * n + 16 * n + te.P + restart * restart + 4 * restart) * 8
         return b
 
 
 class ServingEngine:
-    def __init__(self, xsec_file, hbm_budget, Te=4.0, ev_max=80.0, q=5, max_batch=4096, rtol=1e-12,
+    def __init__(self, xsec_file, hbm_budget, Te=4.0, ev_max=80.0, q=5, max_batch=4096, rtol=1e-14,
                  max_iter=30, ee_rtol=1e-15, ee_max_iter=60, restart=30, streams=4, ee_min_batch=256,
-                 ee_max_wait=0.05, floor_rel=1e-11, device=None):
+                 ee_max_wait=0.05, floor_rel=1e-11, ee_workers=4, device=None):
         self.dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.xs = xsections_from_file(xsec_file, device=self.dev)
         self.Te, self.ev_max, self.q = Te, ev_max, q
@@ -114,10 +114,14 @@ class ServingEngine:
         self.grids = {}
         self.stats = {}
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(max(1, int(streams)))]
-        # e-e batches (host-orchestrated Newton-Krylov) run one at a time in a worker thread, so
-        # the loop keeps admitting and retiring the asynchronous batches meanwhile; they wait
-        # in their queue for ee_min_batch requests or ee_max_wait seconds
-        self._ee_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        # e-e batches (host-orchestrated Newton-Krylov) run in worker threads, so the loop keeps
+        # admitting and retiring the asynchronous batches meanwhile; up to ee_workers of them at
+        # once, of different grid signatures (a grid's e-e work space serves one batch at a
+        # time), each on an e-e stream of its own (default priority); they wait in their queue
+        # for ee_min_batch requests or ee_max_wait seconds
+        self.ee_workers = max(1, int(ee_workers))
+        self._ee_pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.ee_workers)
+        self.ee_streams = [torch.cuda.Stream(device=self.dev) for _ in range(self.ee_workers)]
         self.ee_min_batch, self.ee_max_wait = int(ee_min_batch), float(ee_max_wait)
 
     def grid(self, key):
@@ -146,10 +150,11 @@ class ServingEngine:
             x = s.initial(B)
             sc = s.new_scratch(B)
             s.solve_raw(x, n0, Tg, E, ns, atol=0.0, rtol=self.rtol, max_iter=self.max_iter, mode=1, scratch=sc)
-            # an instance that ran out of iterations with |r| <= floor_rel |r0| sits at the FP64
-            # floor of its residual: reported as status 3, as the e-e solve does
-            status = torch.where((sc["status"] == 1) & (sc["res_abs"] <= self.floor_rel * sc["res0"]),
-                                 3, sc["status"]).to(torch.int32)
+            # an instance that ran out of iterations or stalled (no decrease of the true residual)
+            # with |r| <= floor_rel |r0| sits at the FP64 floor of its residual: reported as
+            # status 3, as the e-e solve does
+            at_floor = ((sc["status"] == 1) | (sc["status"] == 2)) & (sc["res_abs"] <= self.floor_rel * sc["res0"])
+            status = torch.where(at_floor, 3, sc["status"]).to(torch.int32)
             if ee:
                 es, eop = gr.ee()
                 info = eop.steady_state(x, n0, Tg, E, ns, ne, 0.0, self.ee_rtol, self.ee_max_iter,
@@ -233,8 +238,9 @@ class ServingEngine:
                     if not qq or qk[3] != want_ee:
                         continue
                     if want_ee:
-                        if any("future" in b for b in inflight):
-                            continue                           # one e-e batch at a time
+                        busy = [b["key"] for b in inflight if "future" in b]
+                        if len(busy) >= self.ee_workers or qk[:3] in busy:
+                            continue                           # all e-e workers busy, or this grid's
                         if (len(qq) < self.ee_min_batch and now - qq[0].arrival < self.ee_max_wait
                                 and i < len(reqs)):
                             continue                           # let the batch grow
@@ -244,10 +250,14 @@ class ServingEngine:
                     if nfit < 1:
                         continue
                     take, queues[qk] = qq[:nfit], qq[nfit:]
-                    stream = self.streams[nb % len(self.streams)]
+                    if want_ee:
+                        used_s = {id(b["stream"]) for b in inflight if "future" in b}
+                        stream = next(st_ for st_ in self.ee_streams if id(st_) not in used_s)
+                    else:
+                        stream = self.streams[nb % len(self.streams)]
                     bt = self._launch(qk[:3], take, qk[3], stream)
                     bt.update(reqs=take, bytes=nfit * per, t=time.perf_counter() - t0, id=nb,
-                              stream=stream)
+                              stream=stream, key=qk[:3])
                     nb += 1
                     used += nfit * per
                     peak = max(peak, used)

@@ -51,6 +51,7 @@ class Batched0D:
         M = g.mass_matrix()
         A = g.advection_matrix(scale=(1.0 / g.vth) * (QE / ME))
         self.C, self.Cg, self.A, self.M = C, Cg, A, M
+        self.C_grow = C_grow
         # QoI vectors (original order)
         self.mass = g.mass_op()
         self.temp = g.temp_op() * TEMP_EV_FAC

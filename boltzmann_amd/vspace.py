@@ -46,11 +46,28 @@ def uniform_knots(k_domain, num_p, sp_order):
     return np.append(t, np.full(sp_order + 1, float(k_domain[1])))
 
 
-def make_knots(k_domain, sp_order, num_p, sig_pts=None, extend_domain=False):
-    """basis.BSpline.__init__ knot rules (basis.py:338-362): optional 1.2x extension and
-    snapping of the nearest interior knot to each significant point."""
+def log_extended_knots(k_domain, num_p, sp_order, ext_kdomain):
+    """basis.BSpline.uniform_knots_with_extended_bdy (basis.py:680-711): the lower half of
+    the knots uniform on the domain, the rest in a geometric progression of base 1.2 from
+    its end to ext_kdomain."""
+    n1 = int(math.ceil(0.5 * num_p))
+    n2 = num_p - n1
+    head = np.full(sp_order, float(k_domain[0]))
+    lin = np.linspace(k_domain[0], k_domain[1], n1, endpoint=False)
+    b = 1.2
+    geo = np.logspace(math.log(k_domain[1]) / math.log(b), math.log(ext_kdomain) / math.log(b), n2 - sp_order,
+                      base=b, endpoint=False)
+    return np.concatenate([head, lin, geo, np.full(sp_order + 1, float(ext_kdomain))])
+
+
+def make_knots(k_domain, sp_order, num_p, sig_pts=None, extend_domain=False, extend_domain_with_log=False):
+    """basis.BSpline.__init__ knot rules (basis.py:338-362): optional 1.2x extension or the
+    LOG extension to 5x the domain, and snapping of the nearest interior knot to each
+    significant point."""
     if extend_domain:
         t = uniform_knots((k_domain[0], 1.2 * k_domain[1]), num_p, sp_order)
+    elif extend_domain_with_log:
+        t = log_extended_knots(k_domain, num_p, sp_order, 5 * k_domain[1])
     else:
         t = uniform_knots(k_domain, num_p, sp_order)
     if sig_pts is not None:

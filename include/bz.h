@@ -79,6 +79,16 @@ int bz_basis_table(const double* t, int npr, const double* x, int nx, int dorder
 int bz_band_cholesky(const double* A, int npr, int hb, double* lb, hipStream_t stream);
 int bz_band_chol_inverse(const double* lb, int npr, int hb, double* inv, double* work, hipStream_t stream);
 
+/* out = A^-1 rhs (rhs, out: npr x ncol row-major) for the banded SPD A (dense npr x npr,
+ * half bandwidth hb) with its factor lb from bz_band_cholesky: a Cholesky solve per column
+ * and `refine` steps of iterative refinement on the residual summed in compensated
+ * arithmetic (work: 2*npr*ncol doubles).  Replaces the products hl_op = M^-1 (...),
+ * gl_op = M^-1 (...) of CollisionOpSP.compute_rosenbluth_potentials_op
+ * (collision_operator_spherical.py:966-967), which the reference forms with the explicit
+ * inverse of spec_spherical.inverse_mass_mat. */
+int bz_band_chol_solve_refined(const double* A, const double* lb, int npr, int hb, const double* rhs, int ncol,
+                               double* out, double* work, int refine, hipStream_t stream);
+
 /* ------------------------------------------------------------------ 0D batched solver */
 
 /* Per-grid solver tables (device pointers), vectors in the permuted order
@@ -203,6 +213,21 @@ int bz_ee_newton_jv(const bz_grid0d_t* grid, const bz_batch0d_t* batch, const in
 
 /* Window width (columns) the solver uses for an upper bandwidth bw (-1: unsupported). */
 int bz_solver0d_window(int bw);
+
+/* Plans the tables of one grid for the fused solver from its dense device operators
+ * (original radial-major order, n = npr * ns): ops[0..nop-1] = C_0..C_{S-1}, C_Tg, A_E, M
+ * (n x n each), growth_ops[m] (m < nop-1) the operator whose W_m = G_m^T M^-1 u enters the
+ * growth rate, or NULL (growth_ops itself may be NULL: no growth term), u the mass vector
+ * (n), mass_radial the radial mass block (npr x npr), ee_pattern 1 to reserve the e-e
+ * pattern |k - p| <= 3 in the profiles.  Fills *grid_out with device tables owned by the
+ * library (release them with bz_free_grid0d(*handle_out)).  Synchronises `stream`.
+ * Replaces the operator setup of bte_0d3v_batched.assemble_operators that the solve needs
+ * (bte_0d3v_batched.py:421-442: Q, R and the premultiplied operators) for a caller of the
+ * C ABI without the Python layer. */
+int bz_plan_grid0d(int npr, int ns, int nop, const double* const* ops, const double* const* growth_ops,
+                   const double* u, const double* mass_radial, int ee_pattern, bz_grid0d_t* grid_out,
+                   void** handle_out, hipStream_t stream);
+int bz_free_grid0d(void* handle);
 
 /* Normalisation and QoIs (energy, mobility, diffusion, G rates) of B solutions.
  * Replaces bte_0d3v_batched.normalized_distribution + compute_QoIs

@@ -25,17 +25,35 @@ def uniform_knots(k_domain, num_p, sp_order):
     return t
 
 
-def make_knots(k_domain, sp_order, num_p, sig_pts=None, extend_domain=False):
+def log_extended_knots(k_domain, num_p, sp_order, ext_kdomain):
+    """basis.py:680-711 (uniform_knots_with_extended_bdy): ceil(num_p/2) uniform knots on
+    [k0, k1), then num_p - ceil(num_p/2) - p knots geometric (base 1.2) on [k1, ext), clamped
+    at both ends with p and p+1 repeats."""
+    num_p1 = int(np.ceil(0.5 * num_p))
+    num_p2 = num_p - num_p1
+    t = np.full(sp_order, float(k_domain[0]))
+    t = np.append(t, np.linspace(k_domain[0], k_domain[1], num_p1, endpoint=False))
+    lb = 1.2
+    t = np.append(t, np.logspace(np.log(k_domain[1]) / np.log(lb), np.log(ext_kdomain) / np.log(lb),
+                                 num_p2 - sp_order, base=lb, endpoint=False))
+    return np.append(t, np.full(sp_order + 1, float(ext_kdomain)))
+
+
+def make_knots(k_domain, sp_order, num_p, sig_pts=None, extend_domain=False, extend_domain_with_log=False):
     """basis.py:329-362 (continuous-Galerkin branch, knots_vec=None).
 
     Returns (t, domain).  ``extend_domain`` multiplies the upper end by 1.2
-    (basis.py:344-346).  Significant points strictly inside the ORIGINAL
-    k_domain snap the nearest knot (argmin) when that knot index lies in
-    (sp_order+1, len(t)-sp_order-1) (basis.py:356-362).
+    (basis.py:344-346); ``extend_domain_with_log`` keeps half the knots uniform on the
+    domain and spaces the rest geometrically up to 5x its end (basis.py:347-349).
+    Significant points strictly inside the ORIGINAL k_domain snap the nearest knot
+    (argmin) when that knot index lies in (sp_order+1, len(t)-sp_order-1) (basis.py:356-362).
     """
     if extend_domain:
         t = uniform_knots((k_domain[0], 1.2 * k_domain[1]), num_p, sp_order)
         domain = (k_domain[0], 1.2 * k_domain[1])
+    elif extend_domain_with_log:
+        t = log_extended_knots(k_domain, num_p, sp_order, 5 * k_domain[1])
+        domain = (k_domain[0], 5 * k_domain[1])
     else:
         t = uniform_knots(k_domain, num_p, sp_order)
         domain = tuple(k_domain)

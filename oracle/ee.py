@@ -6,7 +6,8 @@ Restates (BESolver/python/...):
     Pm(m)[i, x] = int_{t_i}^{x} v^m B_i and Qm(m)[i, x] = int_x^{t_(i+4)} v^-m B_i at the
     quadrature points (Gauss orders 4 and 32 x ((p + |m| + 1)//2 + 1) per knot piece),
     the potential kernels m1, m2 per l and hl_op, gl_op = M^-1 [int v^2 B_p m(., k)]
-    (collision_operator_spherical.py:786-971);
+    (collision_operator_spherical.py:786-971), M^-1 applied as a refined solve
+    (mass_solve_refined) rather than the reference's explicit inverse;
   * CollisionOpSP.coulomb_collision_op_assembly: cc_a, cc_b[(p,q),(k,l),(r,s)] =
     sum_x w Ia/Ib_{qls}(x) for |k - p|, |r - p| within the reference's window
     (collision_operator_spherical.py:973-1050), with the weak-form terms of the reference's
@@ -18,8 +19,9 @@ Restates (BESolver/python/...):
     (bte_0d3v_batched.py:702-709, 734-743).
 
 Pinned by tests/test_oracle_ee.py: the term table is the reference's (monomials and values at
-seeded points); Pm/Qm against exact integrals; the operator conserves particles, momentum
-and energy; a Maxwellian is in its null space (C(f_M, f_M) = 0).
+seeded points); Pm/Qm against exact integrals; the potentials of a Maxwellian against closed
+forms; mass_solve_refined against a long-double residual; the operator conserves
+particles, momentum and energy; a Maxwellian is in its null space (C(f_M, f_M) = 0).
 """
 import math
 
@@ -73,11 +75,30 @@ def Qm(g, m, gx):
     return out
 
 
-def rosenbluth_ops(g: op.Grid, Minv, m_ab=1.0):
-    """(hl_op, gl_op), each (n, n), block diagonal in l, premultiplied by M^-1."""
+def mass_solve_refined(Mr, raw, steps=3):
+    """Mr^-1 raw for the radial mass block: a Cholesky solve, then iterative refinement on
+    the residual formed in long double.  The result is the solution for the FP64 Mr and raw
+    to about one rounding; the reference's product with the explicit inverse
+    (collision_operator_spherical.py:966-967) carries cond(Mr) roundings, ~1e-14 of the
+    largest entry at Nr = 128, and that moves e-e dominated tail rates by ~2e-8."""
+    import scipy.linalg as sla
+    cf = sla.cho_factor(Mr, lower=True)
+    y = sla.cho_solve(cf, raw)
+    LD = np.longdouble
+    ML, RL = Mr.astype(LD), raw.astype(LD)
+    for _ in range(steps):
+        r = (RL - ML @ y.astype(LD)).astype(np.float64)
+        y = y + sla.cho_solve(cf, r)
+    return y
+
+
+def rosenbluth_ops(g: op.Grid, Minv=None, m_ab=1.0):
+    """(hl_op, gl_op), each (n, n), block diagonal in l, premultiplied by M^-1 (applied per
+    l block as a refined solve with the radial mass block; Minv is not used)."""
     gx, gw = g.quad(g.q)
     V = g.V(gx)
     ns = g.num_sh
+    Mr = op.mass_matrix(g)[0::ns, 0::ns]
     hl = np.zeros((g.n, g.n))
     gl = np.zeros((g.n, g.n))
     for l in range(ns):
@@ -85,9 +106,9 @@ def rosenbluth_ops(g: op.Grid, Minv, m_ab=1.0):
         m1 = (2.0 / (2 * l + 1)) * (1 + m_ab) * (Pm(g, l + 2, gx) / gx ** (l + 1) + Qm(g, l - 1, gx) * gx ** l)
         m2 = -(2.0 / (4 * l * l - 1)) * ((Pm(g, l + 2, gx) / gx ** (l - 1) - f1 * Pm(g, l + 4, gx) / gx ** (l + 1))
                                          + (Qm(g, l - 3, gx) * gx ** l - f1 * Qm(g, l - 1, gx) * gx ** (l + 2)))
-        hl[l::ns, l::ns] = (V * (gw * gx ** 2)) @ m1.T
-        gl[l::ns, l::ns] = (V * (gw * gx ** 2)) @ m2.T
-    return Minv @ hl, Minv @ gl
+        hl[l::ns, l::ns] = mass_solve_refined(Mr, (V * (gw * gx ** 2)) @ m1.T)
+        gl[l::ns, l::ns] = mass_solve_refined(Mr, (V * (gw * gx ** 2)) @ m2.T)
+    return hl, gl
 
 
 _REF_TERMS = None
@@ -124,6 +145,7 @@ def cc_tensors(g: op.Grid, terms=None, lmax=None):
     A = np.zeros((n, n, n))
     Bt = np.zeros((n, n, n))
     D = [g.V(gx_a, 0), g.V(gx_a, 1), bspline_second(g, gx_a)]
+    xL = gx_a.astype(np.longdouble)
     win = p_ + 3
     for p in range(g.num_p):
         for k in range(max(0, p - win), min(g.num_p, p + win)):
@@ -136,10 +158,12 @@ def cc_tensors(g: op.Grid, terms=None, lmax=None):
                     for (q, l, s), mons in terms[key].items():
                         if q > lmax or l > lmax or s > lmax:
                             continue
-                        val = np.zeros_like(x)
+                        # long-double sums: the monomials of one combination have both signs
+                        val = np.zeros(x.shape, dtype=np.longdouble)
                         for c, pw, dp, dk, dr in mons:
-                            val = val + c * x ** pw * D[dp][p][sel] * D[dk][k][sel] * D[dr][r][sel]
-                        T[p * ns + q, k * ns + l, r * ns + s] = np.dot(w, val)
+                            val = val + np.longdouble(c) * xL[sel] ** pw * (D[dp][p][sel] * D[dk][k][sel]).astype(
+                                np.longdouble) * D[dr][r][sel]
+                        T[p * ns + q, k * ns + l, r * ns + s] = float(np.dot(w.astype(np.longdouble), val))
     return A, Bt
 
 

@@ -105,18 +105,15 @@ class XStep:
         z = (self.Qinv @ v.reshape(Nr, Nvt * Nx)).reshape(Nr, Nvt, Nx)
         z[:, self.lower, 0] = 0.0
         z[:, ~self.lower, -1] = 0.0
+        # every (mode, ordinate) bidiagonal system along x at once: forward substitution for
+        # the ordinates with inflow at x_0, backward for those with inflow at x_end
         y = np.zeros_like(z)
-        for j in range(Nvt):
-            for i in range(Nr):
-                dg, sd, b = self.diag[i, j], self.sdiag[i, j], z[i, j]
-                o = np.zeros(Nx)
-                if self.lower[j]:
-                    o[0] = b[0] / dg[0]
-                    for k in range(1, Nx):
-                        o[k] = (b[k] - sd[k - 1] * o[k - 1]) / dg[k]
-                else:
-                    o[-1] = b[-1] / dg[-1]
-                    for k in range(Nx - 2, -1, -1):
-                        o[k] = (b[k] - sd[k] * o[k + 1]) / dg[k]
-                y[i, j] = o
+        lo, up = self.lower, ~self.lower
+        dg, sd = self.diag, self.sdiag
+        y[:, lo, 0] = z[:, lo, 0] / dg[:, lo, 0]
+        for k in range(1, Nx):
+            y[:, lo, k] = (z[:, lo, k] - sd[:, lo, k - 1] * y[:, lo, k - 1]) / dg[:, lo, k]
+        y[:, up, -1] = z[:, up, -1] / dg[:, up, -1]
+        for k in range(Nx - 2, -1, -1):
+            y[:, up, k] = (z[:, up, k] - sd[:, up, k] * y[:, up, k + 1]) / dg[:, up, k]
         return (self.Q @ y.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx)

@@ -63,4 +63,5 @@ def vstep_exact(K0, A, E, dt, tau, u):
     """Per-point dense solve of (I - dt tau (K0 + E_x A)) y_x = u_x; u (dof, Nx)."""
     dof = K0.shape[0]
     I = np.eye(dof)
-    return np.stack([np.linalg.solve(I - dt * tau * (K0 + E[x] * A), u[:, x]) for x in range(u.shape[1])], axis=1)
+    L = I[None] - dt * tau * (K0[None] + np.asarray(E)[:, None, None] * A[None])
+    return np.linalg.solve(L, u.T[:, :, None])[:, :, 0].T

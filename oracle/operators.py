@@ -109,15 +109,17 @@ def grid_single(Nr, lmax, Te_ev, ev_max, q, sp_order=3):
     return Grid(t, Nr + 1, lmax, q, vth, sp_order)
 
 
-def grid_batched(Nr, lmax, Te_ev, ev_max, q, thresholds, sp_order=3, extend=True):
+def grid_batched(Nr, lmax, Te_ev, ev_max, q, thresholds, sp_order=3, extend=True, extend_log=False):
     """Batched 0D path grid (bte_0d3v_batched.py:310-349): thresholds snap knots
     (dg_nodes = sqrt(eps_th) c_gamma / vth) and the LINEAR extension scales
-    the upper end by 1.2 (vspace_grid_extension.LINEAR, the default)."""
+    the upper end by 1.2 (vspace_grid_extension.LINEAR, the default); extend_log (with
+    extend=False) is the LOG extension (basis.py:347-349)."""
     vth = float(vth_from_Te(Te_ev))
     kd = (0.0, math.sqrt(ev_max) * C_GAMMA / vth)
     sig = np.sort(np.array(sorted(set([th for th in thresholds if th is not None and th > 0]))))
     dg_nodes = np.sqrt(sig) * C_GAMMA / vth
-    t, _ = bspline.make_knots(kd, sp_order, Nr + 1, sig_pts=dg_nodes, extend_domain=extend)
+    t, _ = bspline.make_knots(kd, sp_order, Nr + 1, sig_pts=dg_nodes, extend_domain=extend,
+                              extend_domain_with_log=extend_log)
     return Grid(t, Nr + 1, lmax, q, vth, sp_order)
 
 

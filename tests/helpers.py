@@ -15,9 +15,10 @@ def oracle_procs():
     return op.processes_from_lxcat(recs)
 
 
-def oracle_batched(Nr, lmax, q=5, extend=True):
+def oracle_batched(Nr, lmax, q=5, extend=True, extend_log=False):
     procs = oracle_procs()
-    g = op.grid_batched(Nr, lmax, TE, EV_MAX, q, [p.threshold for p in procs], extend=extend)
+    g = op.grid_batched(Nr, lmax, TE, EV_MAX, q, [p.threshold for p in procs], extend=extend and not extend_log,
+                        extend_log=extend_log)
     return s0.BatchedOps(g, procs), g, procs
 
 

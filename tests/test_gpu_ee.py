@@ -225,14 +225,19 @@ def test_steady_state_with_ee_n387():
     oracle's dense exact Newton, stored in tests/golden/ee_n387.npz (make_ee387.py).
 
     The assembly and the solve are checked separately.
-    * Assembly: H, G and the banded tensors against the oracle's.
+    * Assembly: H, G and the banded tensors against the oracle's (entrywise the tensors agree
+      to 2.7e-16 of their largest entry, H and G to 1e-14: H = M_r^-1 raw with cond(M_r) =
+      5.8e6 carries the rounding of raw, which agrees to 1e-15).
     * Solve: with the oracle's H, G and tensors uploaded (K and M stay this library's),
-      energy, mobility and diffusion match to 1e-9 and the rates to 2e-8 (measured 1e-10 and
-      9.8e-9; the rest is the K/M assembly roundoff).
-    * With this library's operators, the ionization rate of the most e-e dominated instance
-      moves by 7.3e-8: an assembly roundoff of 1.4e-14 of max|H| is amplified in that tail.
-      Measured: the oracle run with this library's operators agrees with the GPU solve to
-      9e-11 on every QoI, so the 7.3e-8 belongs to the assembly, not to the solver.
+      energy, mobility and diffusion match to 1e-9 and the rates to 2e-8 (measured 1.7e-10
+      and 1.6e-8).
+    * With this library's operators the most e-e dominated instance (ionisation degree 1e-2)
+      differs by 2.0e-7 on its ionisation rate (energy 2.0e-9); degrees 1e-3, 1e-4, 1e-6 by
+      6.0e-8, 1.3e-8 and 1e-11.  That is this fixed point's FP64 conditioning, not an error
+      of either side: the oracle alone, with its tensors multiplied entrywise by
+      1 + 1e-16 N(0, 1), moves by 2.4e-7 on that rate (energy 2.8e-9).  The oracle's H, G
+      with this library's tensors change the result by less than 1e-9, so the tensors' last
+      bits, which no two FP64 assemblies share, set it.
     """
     import os
     from boltzmann_amd.bte_ee import EEOperator
@@ -276,19 +281,29 @@ def test_steady_state_with_ee_n387():
         print("status", st, "|F|/|F0|", ra / r0, "steps", out["iters"].cpu().numpy(), flush=True)
         assert np.all((st == 0) | ((st == 3) & (ra <= 1e-12 * r0)))
         h, q = s.qois(x, n0, E)
-        return qoi_errors({k: v.cpu().numpy() for k, v in q.items()}, q_ref)
+        qn = {k: v.cpu().numpy() for k, v in q.items()}
+        return qoi_errors(qn, q_ref), qn
 
-    errs_own = solve()
+    def per_instance(q):
+        return [qoi_errors({k: (v[..., i:i + 1]) for k, v in q.items()},
+                           {k: (v[..., i:i + 1]) for k, v in q_ref.items()}) for i in range(4)]
+
+    errs_own, q_own = solve()
     print("qoi errors, this library's operators", errs_own, flush=True)
-    assert max(errs_own[k] for k in ("energy", "mobility", "diffusion")) < 1e-8, errs_own
-    assert errs_own["rates"] < 1e-7, errs_own
-    # the oracle's operators on the device: the solve itself to 1e-8
+    print("  per instance (ion deg 1e-2, 1e-3, 1e-4, 1e-6)", per_instance(q_own), flush=True)
+    # the oracle's H, G with this library's tensors, then all of the oracle's operators
     for l in range(ns_):
         eop.H[l].copy_(_d(hl[l::ns_, l::ns_]))
         eop.G[l].copy_(_d(gl[l::ns_, l::ns_]))
+    errs_hg, q_hg = solve()
+    print("qoi errors, the oracle's H, G", errs_hg, per_instance(q_hg), flush=True)
     eop.ccA.copy_(_d(bands[0]))
     eop.ccB.copy_(_d(bands[1]))
-    errs = solve()
-    print("qoi errors, the oracle's operators", errs, flush=True)
+    errs, q_all = solve()
+    print("qoi errors, the oracle's operators", errs, per_instance(q_all), flush=True)
+    assert max(errs_own[k] for k in ("energy", "mobility", "diffusion")) < 1e-8, errs_own
+    assert errs_own["rates"] < 5e-7, errs_own
+    own = per_instance(q_own)
+    assert own[2]["rates"] < 5e-8 and max(own[3].values()) < 1e-8, own
     assert max(errs[k] for k in ("energy", "mobility", "diffusion")) < 1e-9, errs
     assert errs["rates"] < 2e-8, errs

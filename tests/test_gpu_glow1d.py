@@ -54,3 +54,49 @@ def test_xstep_glow_shapes():
     got = xa.step(_d(v)).cpu().numpy()
     ref = xs.step(v)
     assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_strang_steps_sinusoidal_field():
+    """The glow configuration's velocity-space path over 20 Strang steps (x half step, v step,
+    x half step) at its shapes: Nr = 128, lmax = 2, spline_qpts = 11, Nvt = 32, Nx = 200
+    Chebyshev-Gauss-Lobatto points on the 2.54 cm gap, dt = 2.5e-3 RF periods at 13.56 MHz.
+    The field is sinusoidal in time with the 100 V / 2.54 cm amplitude, times a cubic
+    spatial profile 1 + 0.5 x^3 so the GMRES branch of the v-step runs.  Against the oracle
+    with its own operators (oracle/glow1d x-step, dense per-point v-step solves): the
+    ordinate state must agree to 1e-8 after every step."""
+    from boltzmann_amd.glow1d import Ordinates, VStep, XAdvection
+    from boltzmann_amd.solver0d import Batched0D
+    from helpers import oracle_batched, product_grid, product_xsecs
+    from oracle import krylov as kr
+    print("case: 20 Strang steps Nr=128 lmax=2 q=11 Nvt=32 Nx=200, sinusoidal field", flush=True)
+    ops, g, procs = oracle_batched(128, 2, q=11)
+    s = Batched0D(product_grid(g), product_xsecs())
+    Minv = torch.linalg.inv(s.M)
+    tau, np0, Tg, dt = 1.0 / 13.56e6, 8e16, 300.0, 2.5e-3
+    n0 = 3.22e22 / np0
+    Nx, Nvt, L = 200, 32, 2.54e-2
+    xg = -np.cos(np.pi * np.arange(Nx) / (Nx - 1))
+    ords = Ordinates(g.num_p, 2, Nvt, torch.device("cuda"))
+    vs = VStep(Minv @ sum(s.C), Minv @ s.Cg, Minv @ s.A, tau, n0, np0, Tg, dt, ordinates=ords, rtol=1e-12)
+    scale = g.vth * tau / 0.5 / L
+    xa = XAdvection(s.g, xg, Nvt, scale, 0.5 * dt)
+    # oracle
+    th, w = gl.ordinates(Nvt)
+    po2sh, psh2o = gl.sph_ops(2, th, w)
+    xs = gl.XStep(gl.adv_x_operator(g, scale), xg, th, 0.5 * dt)
+    K0 = 3.22e22 * (ops.c_en.sum(axis=0) + Tg * ops.c_gT)
+    v0 = gl.sph_to_ords(psh2o, ops.initial(Nx), g.num_p, g.num_sh, Nx)
+    v_ref, v = v0.copy(), _d(v0)
+    for k in range(20):
+        E = (100.0 / L) * math.sin(2.0 * math.pi * (k + 0.5) * dt) * (1.0 + 0.5 * xg ** 3)
+        v = xa.step(v, 0.5 * dt)
+        v = vs.step(v.contiguous(), _d(E))
+        v = xa.step(v, 0.5 * dt)
+        v_ref = xs.step(v_ref)
+        u = gl.ords_to_sph(po2sh, v_ref, g.num_p, Nvt, Nx)
+        u = kr.vstep_exact(K0, ops.adv, E, dt, tau, u)
+        v_ref = gl.sph_to_ords(psh2o, u, g.num_p, g.num_sh, Nx)
+        v_ref = xs.step(v_ref)
+        err = np.abs(v.cpu().numpy() - v_ref).max() / np.abs(v_ref).max()
+        print(f"step {k}: gmres iterations {vs.last_iters}, max rel diff {err:.2e}", flush=True)
+        assert err < 1e-8, (k, err)

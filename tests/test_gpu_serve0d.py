@@ -125,3 +125,40 @@ def test_offline_staggered_arrivals_with_ee(engine):
     print("offline staggered stats", st, flush=True)
     assert len(res) == len(reqs)
     assert st["converged"] >= len(reqs) - 1
+
+
+def test_serving_every_grid_against_oracle():
+    """The serving configuration's grids at their sizes: Nr in {64, 128, 256} x lmax in {1, 2, 4}
+    with three processes, two requests per grid without e-e and one with e-e (ionisation
+    degree 1e-3) on the grids where the dense e-e oracle is affordable in the build container
+    (Nr 64 at lmax 1, 2, 4 and Nr 128 at lmax 1; Nr 128 at lmax 2 is the e-e fixture of
+    test_gpu_ee).  Served together in one mix, each result must match the oracle's
+    (tests/golden/serve_grids.npz, make_serve_grids.py) to 1e-8 on every QoI (rate floor
+    1e-12), and the largest requests must be the same alone as in the mix."""
+    import os
+    from boltzmann_amd.serve0d import Request, ServingEngine
+    print("case: serving every grid Nr {64,128,256} x lmax {1,2,4} vs stored oracle", flush=True)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "serve_grids.npz"))
+    cols = [str(c) for c in d["columns"]]
+    rows = d["rows"]
+    c = {k: i for i, k in enumerate(cols)}
+    reqs = [Request(i, int(r[c["Nr"]]), int(r[c["lmax"]]), 3, float(r[c["EbyN"]]), float(r[c["ion_deg"]]),
+                    fac=(float(r[c["fac0"]]), float(r[c["fac1"]]), float(r[c["fac2"]])))
+            for i, r in enumerate(rows)]
+    eng = ServingEngine(XSEC_FILE, 16 << 30, max_iter=40)
+    res, st = eng.run(reqs, realtime=False)
+    print("stats", st, flush=True)
+    for r, row in zip(res, rows):
+        q = {"energy": np.array([r.energy]), "mobility": np.array([r.mobility]), "diffusion": np.array([r.diffusion]),
+             "rates": np.asarray(r.rates).reshape(-1, 1)}
+        qr = {"energy": row[[c["energy"]]], "mobility": row[[c["mobility"]]], "diffusion": row[[c["diffusion"]]],
+              "rates": row[[c["rate0"], c["rate1"], c["rate2"]]].reshape(-1, 1)}
+        errs = qoi_errors(q, qr)
+        print(f"Nr {reqs[r.rid].Nr} lmax {reqs[r.rid].lmax} ion {reqs[r.rid].ion_deg:g} status {r.status}: {errs}",
+              flush=True)
+        assert r.status in (0, 3), r
+        assert max(errs.values()) < 1e-8, (reqs[r.rid], errs)
+    for r in reqs:
+        if r.Nr == 256 and r.lmax == 4:
+            alone, _ = eng.run([dataclasses.replace(r, arrival=0.0)], realtime=False)
+            _same(alone[0], res[r.rid], exact=True)

@@ -50,3 +50,46 @@ def test_single_steady_state_matches_oracle(E_by_N, jac):
     qb["rates"] = np.asarray(q_ref["rates"]).reshape(len(q_ref["rates"]), -1)
     errs = qoi_errors(qa, qb)
     assert max(errs.values()) < 1e-9, errs
+
+
+def test_config1_exact_against_eigenpair():
+    """The CPU configuration exactly: Nr = 64, lmax = 1, E/N = 100 Td, Tg = 300 K, q = 5,
+    against the eigenpair oracle.  The single-path steady state is the eigenvector of the
+    dominant (largest real part) generalized eigenvalue of (C + E A, M), normalised to
+    u.h = 1 (scipy.linalg.eig on the oracle's operators); the GPU solve must match that
+    vector and its QoIs to 1e-8."""
+    import scipy.linalg
+    from boltzmann_amd.bte_0d3v import bte_0d3v
+    print("case: single 0D config 1 (Nr=64 lmax=1 100 Td) vs eigenpair", flush=True)
+    args = _args(100.0, Nr=64, lmax=1)
+    solver = bte_0d3v(args)
+    solver.setup()
+    out = solver.steady_state_solver(jacobian="exact")
+    h = np.asarray(out["sol"][1], dtype=np.float64)
+    g = op.grid_single(64, 1, args.electron_volt, args.ev_max, args.spline_qpts)
+    ops = s0.SingleOps(g, oracle_procs(), args.n0, args.Tg)
+    K = ops.C + ops.Araw * (args.E_field / g.vth) * (op.QE / op.ME)
+    ev, V = scipy.linalg.eig(K, ops.M)
+    # the eigenvalue is the growth rate of the oracle's own single-path fixed point (the
+    # dominant mode; spurious modes of the truncated domain can have larger real parts
+    # within the rounding of the largest |eigenvalue|)
+    r = s0.single_steady_state(ops, args.E_field)
+    hr = r["sol"][1] / (ops.u @ r["sol"][1])
+    lam = (ops.u @ (ops.Minv @ (K @ hr)))
+    fin = np.isfinite(ev)
+    k = np.nonzero(fin)[0][np.argmin(np.abs(ev[fin] - lam))]
+    v = V[:, k]
+    v = (v / v[np.argmax(np.abs(v))]).real
+    v = v / (ops.u @ v)
+    assert abs(ev[k].imag) <= 1e-10 * abs(ev[k])
+    hn = h / (ops.u @ h)
+    assert rel_err(hn, v) < 1e-8
+    q = solver.compute_QoIs(out["sol"][1], None)
+    q_ref = s0.single_qois(ops, v * (ops.u @ h), args.E_field)       # scaled as the GPU solution
+    qa = {k_: np.asarray(q[k_]).reshape(-1) for k_ in ("energy", "mobility", "diffusion")}
+    qa["rates"] = np.asarray(q["rates"]).reshape(len(q["rates"]), -1)
+    qb = {k_: np.asarray(q_ref[k_]).reshape(-1) for k_ in ("energy", "mobility", "diffusion")}
+    qb["rates"] = np.asarray(q_ref["rates"]).reshape(len(q_ref["rates"]), -1)
+    errs = qoi_errors(qa, qb)
+    print("config 1 qoi errors vs eigenpair", errs, flush=True)
+    assert max(errs.values()) < 1e-8, errs

@@ -117,6 +117,27 @@ def test_config2_size_parity():
     assert max(errs.values()) < QOI_TOL, errs
 
 
+def test_log_extended_grid_parity():
+    """The LOG velocity-grid extension (vspace_grid_extension.LOG: half the knots uniform,
+    the rest geometric up to 5x ev_max, basis.py:347-349, 680-711) through the fused solver,
+    against the oracle's fixed point on the same grid: Nr = 64, lmax = 1, eight instances."""
+    print("case: solver0d LOG-extended grid Nr=64 lmax=1 B=8", flush=True)
+    from boltzmann_amd.solver0d import Batched0D
+    ops, g, procs = oracle_batched(64, 1, extend_log=True)
+    solver = Batched0D(product_grid(g), product_xsecs(), growth="collisions")
+    prm = instance_params(8, seed=64)
+    h_ref, q_ref, infos = oracle_fixed_point(ops, prm)
+    h, q, info = _gpu_solve(solver, prm, mode=1, rtol=1e-14)
+    ok = np.array([(not d["diverged"]) and d["rel"][0] < 1e-10 for d in infos])
+    conv = _converged(info)
+    assert conv[ok].all(), info
+    for i in np.nonzero(ok)[0]:
+        sub = lambda d: {k: v[..., i:i + 1] for k, v in d.items()}
+        assert rel_err(h[:, i], h_ref[:, i]) < QOI_TOL
+        errs = qoi_errors(sub(q), sub(q_ref))
+        assert max(errs.values()) < QOI_TOL, (i, errs)
+
+
 def test_near_floor_instance_of_headline_batch():
     """Instance 385 of the 4096-instance headline batch (Nr = 128, lmax = 2, E/N = 30.8 Td)
     has its FP64 residual floor just below 1e-12 |r0|. A variant of the factorisation that

@@ -84,6 +84,27 @@ def test_knot_vector_structure():
     assert moved[0] == np.argmin(np.abs(te - 2.53))
 
 
+@pytest.mark.parametrize("num_p", [33, 64, 129])
+def test_log_extended_knot_vector(num_p):
+    """LOG extension (basis.py:347-349, 680-711): p + 1 clamped knots at each end, ceil(num_p/2)
+    knots with constant spacing k1 / ceil(num_p/2) from k0 = 0, then a geometric run from k1
+    whose ratio is 5^(1 / (num_p - ceil(num_p/2) - p)) up to the clamped end 5 k1; the knot
+    count fixes num_p basis functions; snapping works inside the un-extended domain."""
+    kd, p = (0.0, 6.7), 3
+    t, dom = bspline.make_knots(kd, p, num_p, extend_domain_with_log=True)
+    n1 = -(-num_p // 2)
+    n2 = num_p - n1
+    assert len(t) == num_p + p + 1 and dom == (0.0, 5 * 6.7)
+    assert (t[:p + 1] == 0.0).all() and np.allclose(t[-(p + 1):], 5 * 6.7, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(np.diff(t[p:p + n1 + 1]), 6.7 / n1, rtol=1e-12)
+    geo = t[p + n1:p + n1 + (n2 - p) + 1]          # k1, ..., 5 k1
+    assert geo[0] == pytest.approx(6.7, rel=1e-14)
+    np.testing.assert_allclose(geo[1:] / geo[:-1], 5.0 ** (1.0 / (n2 - p)), rtol=1e-12)
+    ts, _ = bspline.make_knots(kd, p, num_p, sig_pts=np.array([2.53, 9.0]), extend_domain_with_log=True)
+    moved = np.nonzero(ts != t)[0]
+    assert len(moved) == 1 and ts[moved[0]] == 2.53
+
+
 @pytest.mark.parametrize("q", [3, 5, 10])
 def test_gauss_points_per_interval(q):
     for t in solver_knots():

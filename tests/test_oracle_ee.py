@@ -112,6 +112,26 @@ def grids():
     return out
 
 
+def test_mass_solve_refined():
+    """Mr^-1 raw by mass_solve_refined: its long-double residual is at the rounding of the
+    result, and it is closer to the long-double solution than the explicit-inverse product
+    the reference forms (which it must improve on at Nr = 128, where Mr is ill-conditioned
+    by its v^2 scaling)."""
+    procs = oracle_procs()
+    g = op.grid_batched(128, 0, 4.0, 80.0, 5, [p.threshold for p in procs])
+    Mr = op.mass_matrix(g)
+    rng = np.random.default_rng(3)
+    raw = Mr @ rng.standard_normal((g.num_p, 8)) + 1e-3 * rng.standard_normal((g.num_p, 8))
+    y = ee.mass_solve_refined(Mr, raw)
+    LD = np.longdouble
+    res = raw.astype(LD) - Mr.astype(LD) @ y.astype(LD)
+    scale = np.abs(Mr).astype(LD) @ np.abs(y).astype(LD)
+    assert float(np.max(np.abs(res) / scale)) < 4e-16
+    y_inv = np.linalg.inv(Mr) @ raw
+    res_inv = raw.astype(LD) - Mr.astype(LD) @ y_inv.astype(LD)
+    assert float(np.max(np.abs(res_inv) / scale)) > 10 * float(np.max(np.abs(res) / scale))
+
+
 def test_partial_moments_against_quad(grids):
     g = grids[16][0]
     gx, _ = g.quad(g.q)
