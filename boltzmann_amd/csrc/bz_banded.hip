@@ -167,6 +167,65 @@ __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
   }
 }
 
+// The x-step's bidiagonal upwind solves as a parallel scan: one wavefront per system.  Along
+// the sweep the solution obeys o_p = a_p o_(p-1) + b_p (a_p = -s_p / d_p, b_p = rhs_p / d_p,
+// a_0 = 0 and rhs_0 = 0: the inflow value is zeroed), a composition of affine maps.  Each
+// lane composes the maps of PER consecutive positions, the 64 lane maps are combined by a
+// shuffle scan (6 steps), and each lane then re-runs its positions from the value entering
+// its segment.  Every map has |a_p| < 1 (upwind, diagonally dominant), so the re-association
+// changes the result only at rounding level.  The sequential sweep needed N dependent steps
+// per system on 64 systems per wavefront; this takes N / 64 per system.
+template <int PER>
+__global__ __launch_bounds__(64) void k_xstep_scan(const double* __restrict__ diag, const double* __restrict__ sdiag,
+                                                   const int* __restrict__ lower_of, const double* __restrict__ rhs,
+                                                   double* __restrict__ out, int bsz, int Nvt, int N) {
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (s >= bsz) return;
+  const bool fwd = lower_of[s % Nvt] != 0;
+  const size_t row = (size_t)s * N;
+  const size_t srow = (size_t)s * (N - 1);
+  double al[PER], be[PER];
+  double A = 1.0, B = 0.0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int p = lane * PER + q;                 // position along the sweep
+    double a = 1.0, b = 0.0;                      // identity beyond the end
+    if (p < N) {
+      const int c = fwd ? p : N - 1 - p;          // matrix row
+      const double inv = 1.0 / diag[row + c];
+      const int sc = fwd ? c - 1 : c;             // coupling to the upwind neighbour
+      const double sv = (p >= 1) ? sdiag[srow + sc] : 0.0;
+      const double r = (p >= 1) ? rhs[row + c] : 0.0;
+      a = -sv * inv;
+      b = r * inv;
+    }
+    al[q] = a;
+    be[q] = b;
+    B = fma(a, B, b);
+    A = a * A;
+  }
+  // inclusive scan of the lane maps: (A, B) <- (A, B) o (A_prev, B_prev)
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ap = __shfl_up(A, off, 64);
+    const double Bp = __shfl_up(B, off, 64);
+    if (lane >= off) {
+      B = fma(A, Bp, B);
+      A = A * Ap;
+    }
+  }
+  // value entering this lane's segment: the end of the previous lane's (position 0 has a = 0)
+  double o = __shfl_up(B, 1, 64);
+  if (lane == 0) o = 0.0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int p = lane * PER + q;
+    o = fma(al[q], o, be[q]);
+    if (p < N) out[row + (fwd ? p : N - 1 - p)] = o;
+  }
+}
+
 template <int KIND>
 int launch_sweep(const SweepArgs& a, int nd, hipStream_t st) {
   const dim3 grid((a.bsz + TS - 1) / TS), block(TS);
@@ -282,7 +341,16 @@ int bz_xstep_bidiag_solve(const double* diag, const double* subdiag, const int* 
   SweepArgs a{};
   a.diag = diag; a.sdiag = subdiag; a.lower_of = lower_of; a.b = b; a.out = out;
   a.bsz = Nr * Nvt; a.N = Nx; a.Nr = Nr; a.Nvt = Nvt; a.Nx = Nx;
-  return launch_sweep<K_XSTEP>(a, 2, stream);
+  const int bsz = Nr * Nvt;
+  const int per = (Nx + 63) / 64;
+  if (per > 16) return launch_sweep<K_XSTEP>(a, 2, stream);       // long lines: the chunked sweep
+  const dim3 grid(bsz), block(64);
+  if (per <= 2) hipLaunchKernelGGL((k_xstep_scan<2>), grid, block, 0, stream, diag, subdiag, lower_of, b, out, bsz, Nvt, Nx);
+  else if (per <= 4) hipLaunchKernelGGL((k_xstep_scan<4>), grid, block, 0, stream, diag, subdiag, lower_of, b, out, bsz, Nvt, Nx);
+  else if (per <= 8) hipLaunchKernelGGL((k_xstep_scan<8>), grid, block, 0, stream, diag, subdiag, lower_of, b, out, bsz, Nvt, Nx);
+  else hipLaunchKernelGGL((k_xstep_scan<16>), grid, block, 0, stream, diag, subdiag, lower_of, b, out, bsz, Nvt, Nx);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
 }
 
 int bz_banded_tri_solve_batched(const double* diags, const double* b, double* out, int bsz, int ndiags, int N,

@@ -135,6 +135,148 @@ class XAdvection:
         return (self.Q @ y.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx)
 
 
+class _DeviceGMRES:
+    """The v-step system (I - dt tau (K0 + E_x A)) y = u, right-preconditioned by the E-bin
+    block inverses, by restarted GMRES with the whole Arnoldi step on the device.
+
+    The same Krylov method as utils.gmres_pcr (utils.py:1172-1268: right preconditioning,
+    x0 = u, restart, tol = max(atol, rtol |b|)), with two changes that remove the host from
+    the inner loop: the residual of a step is the least-squares residual |g_(i+1)| of the
+    Givens-rotated Hessenberg system (equal to the true residual in exact arithmetic), and
+    the true residual is evaluated once per cycle, at its end.  The cycle is replayed from
+    HIP graphs of `chunk` steps each (captured on first use); the host reads the device stop
+    flag between chunks.  Per step: the binned preconditioner (bz_binned_apply), the operator
+    product (bz_vstep_matvec), classical Gram-Schmidt twice (bz_multi_dot, bz_vt_axpy), the
+    norm, the scaling and one Givens rotation (bz_gmres_givens): 9 launches, none of them
+    reading back to the host."""
+
+    def __init__(self, vs, nx, chunk=4):
+        dev = vs.dev
+        self.vs, self.nx, self.dof = vs, nx, vs.dof
+        self.n = n = vs.dof * nx
+        self.m = m = vs.rsrt
+        self.chunk = chunk
+        f64, i32 = torch.float64, torch.int32
+        self.nblk = max(1, min(1024, (n + 4095) // 4096))
+        self.V = torch.empty((m + 1, n), dtype=f64, device=dev)
+        self.H = torch.zeros((m + 2, m + 1), dtype=f64, device=dev)
+        self.ldh = m + 1
+        self.partial = torch.empty(((m + 1) * self.nblk,), dtype=f64, device=dev)
+        self.h = torch.zeros(m + 2, dtype=f64, device=dev)
+        self.cs = torch.zeros(m + 1, dtype=f64, device=dev)
+        self.sn = torch.zeros(m + 1, dtype=f64, device=dev)
+        self.g = torch.zeros(m + 2, dtype=f64, device=dev)
+        self.y = torch.zeros(m + 1, dtype=f64, device=dev)
+        self.tol = torch.zeros(1, dtype=f64, device=dev)
+        self.scal = torch.zeros(4, dtype=f64, device=dev)
+        self.flags = torch.zeros(2, dtype=i32, device=dev)          # done, kdone
+        self.w = torch.empty(n, dtype=f64, device=dev)
+        self.t = torch.empty(n, dtype=f64, device=dev)
+        self.x = torch.empty(n, dtype=f64, device=dev)
+        self.b = torch.empty(n, dtype=f64, device=dev)
+        self.r = torch.empty(n, dtype=f64, device=dev)
+        self.E = torch.empty(nx, dtype=f64, device=dev)
+        self.nbins = vs.PmatE.shape[0]
+        self.bins = torch.empty(nx, dtype=i32, device=dev)
+        self.gstart = torch.empty(self.nbins, dtype=i32, device=dev)
+        self.gcount = torch.empty(self.nbins, dtype=i32, device=dev)
+        self.order = torch.empty(nx, dtype=i32, device=dev)
+        self.graphs = None
+        self.c = vs.dt * vs.tau
+
+    def _done(self):
+        return _ptr(self.flags, 0)
+
+    def _precond(self, src, dst, done=None):
+        _lib.call("bz_binned_apply", self.vs.PmatE, self.dof, self.nbins, self.gstart, self.gcount, self.order,
+                  src, self.nx, dst, done)
+
+    def _matvec(self, src, dst, done=None):
+        _lib.call("bz_vstep_matvec", self.vs.K0, self.vs.A, src, self.E, self.c, self.dof, self.nx, dst, done)
+
+    def _dot(self, V, nv, u, out, ostride=1, accumulate=0, take_sqrt=0, out2=None):
+        _lib.call("bz_multi_dot", _ptr(V), self.n, nv, _ptr(u), self.n, self.partial, self.nblk, out, ostride,
+                  accumulate, take_sqrt, out2)
+
+    def _step(self, i):
+        n, ldh = self.n, self.ldh
+        done = self._done()
+        self._precond(_ptr(self.V, i * n), self.t, done)
+        self._matvec(self.t, self.w, done)
+        for rep in range(2):
+            self._dot(self.V, i + 1, self.w, _ptr(self.H, i), ostride=ldh, accumulate=rep, out2=self.h)
+            _lib.call("bz_vt_axpy", _ptr(self.V), n, i + 1, self.h, 1, -1.0, self.w, self.w, n)
+        self._dot(self.w, 1, self.w, _ptr(self.H, (i + 1) * ldh + i), take_sqrt=1)
+        _lib.call("bz_scale_inv", self.w, _ptr(self.H, (i + 1) * ldh + i), _ptr(self.V, (i + 1) * n), n)
+        _lib.call("bz_gmres_givens", self.H, ldh, i, self.cs, self.sn, self.g, self.tol, _ptr(self.flags, 0),
+                  _ptr(self.flags, 1))
+
+    def _capture(self):
+        self.graphs = []
+        for i0 in range(0, self.m, self.chunk):
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr):
+                for i in range(i0, min(self.m, i0 + self.chunk)):
+                    self._step(i)
+            self.graphs.append(gr)
+
+    def _cycle_start(self, atol, rtol):
+        """r = b - L P x, |r| -> scal[0], V[0] = r / |r|, H = 0, g, tol and the flags set."""
+        self._precond(self.x, self.t)
+        self._matvec(self.t, self.r)
+        torch.sub(self.b, self.r, out=self.r)
+        self._dot(self.r, 1, self.r, _ptr(self.scal, 0), take_sqrt=1)
+        self.H.zero_()
+        _lib.call("bz_gmres_cycle_start", self.scal, float(atol), float(rtol), self.m, self.g, self.tol,
+                  _ptr(self.flags, 0), _ptr(self.flags, 1))
+        _lib.call("bz_scale_inv", self.r, self.scal, _ptr(self.V, 0), self.n)
+
+    def solve(self, u, E, max_cycles=50):
+        vs = self.vs
+        self.E.copy_(E)
+        _lib.call("bz_nearest_bin", self.E, self.nx, vs.Evals, vs.Evals.shape[0], self.bins)
+        _lib.call("bz_group_bins", self.bins, self.nx, self.nbins, self.gstart, self.gcount, self.order)
+        self.b.copy_(u.reshape(-1))
+        self.x.copy_(self.b)                                       # x0 = u, as the reference
+        self._dot(self.b, 1, self.b, _ptr(self.scal, 1), take_sqrt=1)
+        first = self.graphs is None
+        iters = 0
+        for cyc in range(max_cycles):
+            self._cycle_start(vs.atol, vs.rtol)
+            if first:
+                # the first cycle runs eagerly (every kernel loaded and run once), then the
+                # steps are captured for the later cycles and v-steps
+                for i in range(self.m):
+                    self._step(i)
+                    if (i + 1) % self.chunk == 0 and int(self.flags[0].item()):
+                        break
+                self._capture()
+                first = False
+            else:
+                for gr in self.graphs:
+                    gr.replay()
+                    if int(self.flags[0].item()):
+                        break
+            k = int(self.flags[1].item())
+            iters += k
+            _lib.call("bz_gmres_back_solve", self.H, self.ldh, self.g, _ptr(self.flags, 1), self.y, self.m)
+            _lib.call("bz_vt_axpy", _ptr(self.V), self.n, self.m, self.y, 1, 1.0, self.x, self.x, self.n)
+            # the true residual of the cycle's end
+            self._precond(self.x, self.t)
+            self._matvec(self.t, self.r)
+            torch.sub(self.b, self.r, out=self.r)
+            self._dot(self.r, 1, self.r, _ptr(self.scal, 2), take_sqrt=1)
+            rn, tol = float(self.scal[2].item()), float(self.tol[0].item())
+            if rn < tol:
+                return self.t.reshape(self.dof, self.nx).clone(), iters
+        raise RuntimeError(f"v-step GMRES did not converge in {max_cycles} cycles of {self.m}")
+
+
+def _ptr(t, offset=0):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr() + t.element_size() * offset)
+
+
 def default_pc_fields(num_pc_evals=30):
     """Preconditioner field values of the reference set-up (:2087-2092): +-logspace(1e2, 6e5)."""
     ep = np.logspace(np.log10(1e2), np.log10(6e5), num_pc_evals // 2, base=10)
@@ -172,6 +314,7 @@ class VStep:
         mats.append(I - c * Ev[-1] * self.A - c * self.K0)
         self.PmatE = torch.linalg.inv(torch.stack(mats)).contiguous()     # (len(Ev)+1, dof, dof), set-up only
         self.last_iters = 0
+        self._gm = {}
 
     def _lmat(self, E):
         nx = E.shape[0]
@@ -217,14 +360,25 @@ class VStep:
             return y.reshape(-1)
         return mv
 
-    def solve_sph(self, u, E):
-        """u (dof, Nx) in the harmonic basis, E (Nx,) device -> y (dof, Nx)."""
+    def solve_sph(self, u, E, use_graph=True):
+        """u (dof, Nx) in the harmonic basis, E (Nx,) device -> y (dof, Nx).
+
+        use_graph: the device GMRES (_DeviceGMRES: MFMA products, device residual estimate,
+        restart cycles replayed from HIP graphs); otherwise krylov.gmres_pcr, the reference's
+        loop with a host-checked true residual after every step."""
         from .krylov import gmres_pcr
         E = E.contiguous().to(torch.float64)
         nx = E.shape[0]
         if bool((E == E[0]).all()):
             L = torch.eye(self.dof, dtype=torch.float64, device=self.dev) - self.dt * self.tau * (self.K0 + E[0] * self.A)
             return torch.linalg.inv(L) @ u
+        if use_graph:
+            g = self._gm.get(nx)
+            if g is None:
+                g = self._gm[nx] = _DeviceGMRES(self, nx)
+            y, iters = g.solve(u, E)
+            self.last_iters = iters
+            return y
         counter = [0]
 
         def cb(_):

@@ -352,6 +352,29 @@ int bz_nearest_bin(const double* E, int nx, const double* Ev, int nb, int* bin, 
 int bz_binned_block_apply(const double* P, int dof, const int* bin, const double* u, int nx, double* y,
                           hipStream_t stream);
 
+/* The v-step GMRES with every Arnoldi step on the device (no host synchronisation inside a
+ * restart cycle, so a cycle can be captured in a HIP graph).  Replaces the dense products and
+ * the per-point preconditioner of step_bte_v (glowdischarge_boltzmann_1d.py:1476-1592) and
+ * the host-side least squares and residual tests of utils.gmres_pcr (utils.py:1172-1268):
+ *   bz_vstep_matvec:   Y = Z - c (K0 Z + E_x (A Z)) on the FP64 matrix cores, Z, Y (dof x nx);
+ *   bz_group_bins:     the points grouped by their E bin (counting sort, stable);
+ *   bz_binned_apply:   Y[:, x] = P[bin(x)] Z[:, x] for all bins in one launch (P: nbins blocks);
+ *   bz_gmres_cycle_start, bz_gmres_givens, bz_gmres_back_solve: the residual estimate of a
+ *   cycle by Givens rotations, a device stop flag (done, kdone) and the final triangular solve.
+ * `done` may be NULL for matvec / binned_apply outside a cycle; inside one, a set flag makes
+ * every step a no-op. */
+int bz_vstep_matvec(const double* K0, const double* A, const double* Z, const double* E, double c, int dof, int nx,
+                    double* Y, const int* done, hipStream_t stream);
+int bz_group_bins(const int* bin, int nx, int nbins, int* grp_start, int* grp_count, int* order, hipStream_t stream);
+int bz_binned_apply(const double* P, int dof, int nbins, const int* grp_start, const int* grp_count,
+                    const int* order, const double* Z, int nx, double* Y, const int* done, hipStream_t stream);
+int bz_gmres_cycle_start(const double* scal, double atol, double rtol, int m, double* g, double* tol, int* done,
+                         int* kdone, hipStream_t stream);
+int bz_gmres_givens(double* H, int ldh, int i, double* cs, double* sn, double* g, const double* tol, int* done,
+                    int* kdone, hipStream_t stream);
+int bz_gmres_back_solve(const double* H, int ldh, const double* g, const int* kdone, double* y, int m,
+                        hipStream_t stream);
+
 /* y = x - c (Kx + E[x] Ax) over (dof, nx) arrays: Lmat_mvec of step_bte_v
  * (glowdischarge_boltzmann_1d.py:1520-1524) from the two operator products, c = dt tau. */
 int bz_vstep_combine(const double* x, const double* Kx, const double* Ax, const double* E, double c, int dof, int nx,
