@@ -394,3 +394,63 @@ class VStep:
         """v (num_p*Nvt, Nx) ordinates -> ordinates after one BE v-step."""
         u = self.ords.ords_to_sph(v)
         return self.ords.sph_to_ords(self.solve_sph(u, E).contiguous())
+
+
+class glow1d_vspace:
+    """The velocity-space methods of glowdischarge_boltzmann_1d.glow1d_boltzmann with the
+    reference's names and signatures, for a 1D3V time loop that keeps its own x-space and
+    field solve: ords_to_sph / sph_to_ords (:1032-1048), initialize_bte_adv_x (:718-834),
+    step_bte_x (:1220-1287) and step_bte_v (:1476-1592, ts_type "BE").
+
+    op_col_en, op_col_gT, op_adv_v: the v-space operators premultiplied by M^-1, (dof_v,
+    dof_v) device tensors, as the reference holds them; vgrid: the radial VGrid; xp: the Np
+    spatial points on [-1, 1]; Nvt: ordinates; param: tau, n0, np0, Tg and L (gap length);
+    args: rtol, atol and gmres_rsrt of the v-step GMRES.  The caller sets bs_E (Np,) on the
+    device before step_bte_v, as the reference's loop does.  The v-step is the device GMRES
+    (VStep); its E-bin preconditioner table is built for each new dt."""
+
+    def __init__(self, op_col_en, op_col_gT, op_adv_v, vgrid, xp, Nvt, param, args):
+        self.op_col_en, self.op_col_gT, self.op_adv_v = op_col_en, op_col_gT, op_adv_v
+        self.vgrid = vgrid
+        self.xp = np.asarray(xp, dtype=np.float64)
+        self.Np, self.Nvt, self.Nr = len(self.xp), int(Nvt), vgrid.npr
+        self.dof_v = op_col_en.shape[0]
+        self.param, self.args = param, args
+        self.ords = Ordinates(vgrid.npr, vgrid.lmax, self.Nvt, vgrid.dev)
+        self.xp_vt, _ = ordinates(self.Nvt)
+        self.adv_setup_dt = None
+        self.bs_E = None
+        self._xadv = None
+        self._vsteps = {}
+
+    def ords_to_sph(self, x):
+        return self.ords.ords_to_sph(x)
+
+    def sph_to_ords(self, x):
+        return self.ords.sph_to_ords(x)
+
+    def initialize_bte_adv_x(self, dt):
+        """x-advection operators of a step of length dt (RF periods): the eigen basis of
+        M^-1 (vth tau / (L/2)) [int v^3 B_p B_k] and the upwind backward-Euler factors."""
+        scale = self.vgrid.vth * self.param.tau / (0.5 * self.param.L)
+        self._xadv = XAdvection(self.vgrid, self.xp, self.Nvt, scale, dt)
+        self.adv_setup_dt = dt
+
+    def step_bte_x(self, v, time, dt, verbose=0):
+        """v (Nr*Nvt, Np) ordinates -> advected over dt (the step the operators were set up for)."""
+        assert self.adv_setup_dt == dt
+        return self._xadv.step(v, dt)
+
+    def step_bte_v(self, v, dv, time, dt, ts_type, verbose=0):
+        """v (Nr*Nvt, Np) ordinates -> one backward-Euler v-step under the field bs_E."""
+        if ts_type != "BE":
+            raise NotImplementedError(f"ts_type {ts_type!r}: the v-step of this library is backward Euler")
+        vs = self._vsteps.get(dt)
+        if vs is None:
+            p = self.param
+            vs = VStep(self.op_col_en, self.op_col_gT, self.op_adv_v, p.tau, p.n0, p.np0, p.Tg, dt,
+                       ordinates=self.ords, rtol=self.args.rtol, atol=self.args.atol,
+                       gmres_rsrt=self.args.gmres_rsrt)
+            self._vsteps[dt] = vs
+        E = torch.as_tensor(self.bs_E, dtype=torch.float64, device=self.vgrid.dev).reshape(-1)
+        return vs.step(v.contiguous(), E)

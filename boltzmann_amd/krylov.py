@@ -3,8 +3,9 @@
 Mirrors utils.gmres_pcr (BESolver/python/utils.py:1172-1268): same arguments, same
 right preconditioning (A M^-1 y = b, x = M^-1 y), the least-squares problem of the
 Hessenberg matrix solved after every Arnoldi step, a true-residual check b - A M^-1 x_n
-after every step, tol = max(atol, rtol |b|), return value (solution, info) with info 0
-on convergence and the iteration count otherwise.
+after every step, tol = max(atol, rtol |b|), maxiter tested between restart cycles only (a
+cycle always runs to its end), return value (solution, info) with info 0 on convergence and
+the iteration count otherwise.
 
 Design: the Krylov basis lives in HBM as rows of one (restart+1, n) array; every
 Arnoldi step is two block dot products and two block updates (classical Gram-Schmidt
@@ -64,7 +65,7 @@ def gmres_pcr(A, b, x0=None, atol=1e-20, rtol=1e-10, maxiter=None, restart=None,
         maxiter = n * 10
     if restart is None:
         restart = maxiter
-    restart = int(min(restart, maxiter))
+    restart = int(restart)
     w = _Work(n, restart, b.device)
     b = b.contiguous()
     b_norm = float(w.norm(b).item())
@@ -102,7 +103,5 @@ def gmres_pcr(A, b, x0=None, atol=1e-20, rtol=1e-10, maxiter=None, restart=None,
             if rn_norm < tol:
                 return psolve(xn), 0
             it += 1
-            if it >= maxiter:
-                break
         x = xn
     return psolve(xn), it + 1

@@ -100,3 +100,39 @@ def test_strang_steps_sinusoidal_field():
         err = np.abs(v.cpu().numpy() - v_ref).max() / np.abs(v_ref).max()
         print(f"step {k}: gmres iterations {vs.last_iters}, max rel diff {err:.2e}", flush=True)
         assert err < 1e-8, (k, err)
+
+
+def test_glow1d_vspace_dropin_methods():
+    """The reference-signature methods (initialize_bte_adv_x, step_bte_x, step_bte_v with
+    bs_E set by the caller) drive the same x-v-x step as the classes underneath: three Strang
+    steps at the glow shapes against XAdvection + VStep called directly."""
+    import argparse
+    from boltzmann_amd.glow1d import Ordinates, VStep, XAdvection, glow1d_vspace
+    from boltzmann_amd.solver0d import Batched0D
+    from helpers import oracle_batched, product_grid, product_xsecs
+    print("case: glow1d_vspace drop-in methods, 3 Strang steps", flush=True)
+    ops, g, procs = oracle_batched(128, 2, q=11)
+    s = Batched0D(product_grid(g), product_xsecs())
+    Minv = torch.linalg.inv(s.M)
+    C_en, C_gT, A_v = Minv @ sum(s.C), Minv @ s.Cg, Minv @ s.A
+    tau, np0, Tg, dt, L = 1.0 / 13.56e6, 8e16, 300.0, 5e-5, 2.54e-2
+    Nx, Nvt = 200, 32
+    xg = -np.cos(np.pi * np.arange(Nx) / (Nx - 1))
+    param = argparse.Namespace(tau=tau, n0=3.22e22 / np0, np0=np0, Tg=Tg, L=L)
+    args = argparse.Namespace(rtol=1e-12, atol=1e-20, gmres_rsrt=30)
+    gv = glow1d_vspace(C_en, C_gT, A_v, s.g, xg, Nvt, param, args)
+    gv.initialize_bte_adv_x(0.5 * dt)
+    ords = Ordinates(g.num_p, 2, Nvt, torch.device("cuda"))
+    vs = VStep(C_en, C_gT, A_v, tau, param.n0, np0, Tg, dt, ordinates=ords, rtol=1e-12)
+    xa = XAdvection(s.g, xg, Nvt, g.vth * tau / (0.5 * L), 0.5 * dt)
+    v = ords.sph_to_ords(s.initial(Nx).contiguous())
+    w = v.clone()
+    for k in range(3):
+        E = _d(1e4 * xg ** 3 * (1.0 + 0.1 * k))
+        gv.bs_E = E
+        v = gv.step_bte_x(v, k * dt, 0.5 * dt)
+        v = gv.step_bte_v(v, None, k * dt, dt, "BE")
+        v = gv.step_bte_x(v, k * dt, 0.5 * dt)
+        w = xa.step(vs.step(xa.step(w, 0.5 * dt).contiguous(), E), 0.5 * dt)
+        err = float((v - w).abs().max() / w.abs().max())
+        assert err < 1e-12, (k, err)

@@ -53,6 +53,24 @@ def test_gmres_dense_system(n, restart, precond):
     assert np.linalg.norm(xo - ref) <= 1e-9 * np.linalg.norm(ref)
 
 
+def test_gmres_maxiter_between_cycles():
+    """maxiter is tested between restart cycles only (utils.py:1214-1268): with restart 10 and
+    maxiter 3 on a system that needs more, one full cycle of 10 steps runs and the return value
+    is (the cycle's iterate, 11), as in the oracle's restatement; through the utils mirror
+    with the reference's signature (xp first)."""
+    from boltzmann_amd.utils import gmres_pcr
+    print("case: gmres maxiter semantics n=400 restart=10 maxiter=3", flush=True)
+    n = 400
+    rng = np.random.default_rng(4)
+    A = np.eye(n) + 2.0 * rng.normal(size=(n, n)) / math.sqrt(n)
+    b = rng.normal(size=n)
+    Ad = _d(A)
+    x, info = gmres_pcr(None, lambda v: Ad @ v, _d(b), rtol=1e-12, restart=10, maxiter=3)
+    xo, info_o = kr.gmres_pcr(lambda v: A @ v, b, rtol=1e-12, restart=10, maxiter=3)
+    assert info == info_o == 11
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
 def test_vstep_glow_shapes():
     from boltzmann_amd.glow1d import Ordinates, VStep
     from boltzmann_amd.solver0d import Batched0D
