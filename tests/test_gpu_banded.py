@@ -118,3 +118,30 @@ def test_bsr_upper_unit_solve(bsz, nrhs):
     got = s.solve(_d(b))
     torch.cuda.synchronize()
     _close(got, ref)
+
+
+def test_spsm_solver_mirror():
+    """cusparse_spsolve.SpSMSolver's interface (scipy BSR matrix in, repeated solve(b)) over
+    the BSR kernel: the unit-diagonal block upper triangle of a random block matrix against
+    scipy.linalg.solve_triangular on the dense upper triangle with unit diagonal."""
+    import scipy.linalg
+    import scipy.sparse
+    from boltzmann_amd.cusparse_spsolve import SpSMSolver, solver_type
+    print("case: SpSMSolver mirror bsz=8 nbr=16", flush=True)
+    rng = np.random.default_rng(9)
+    bsz, nbr = 8, 16
+    A = scipy.sparse.random(nbr * bsz, nbr * bsz, density=0.2, random_state=3, format="csr") * 0.2
+    A = scipy.sparse.bsr_matrix(A.toarray() + np.eye(nbr * bsz), blocksize=(bsz, bsz))
+    s = SpSMSolver(A, nrhs=4, lower=False, unit_diagonal=True, stype=solver_type.CUDA_CODE)
+    # the kernel uses the blocks strictly above the diagonal; the diagonal blocks are identity
+    dense = A.toarray()
+    U = np.eye(nbr * bsz)
+    for i in range(nbr):
+        for j in range(i + 1, nbr):
+            U[i * bsz:(i + 1) * bsz, j * bsz:(j + 1) * bsz] = dense[i * bsz:(i + 1) * bsz, j * bsz:(j + 1) * bsz]
+    for nrhs in (1, 4):
+        b = rng.normal(size=(nbr * bsz, nrhs)) if nrhs > 1 else rng.normal(size=nbr * bsz)
+        ref = scipy.linalg.solve_triangular(U, b, lower=False, unit_diagonal=True)
+        _close(s.solve(b), ref)
+    with pytest.raises(NotImplementedError):
+        SpSMSolver(A, lower=True)
