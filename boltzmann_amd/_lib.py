@@ -45,6 +45,14 @@ class EE0D(ctypes.Structure):
                 ("negam", c_void_p), ("h_lo", c_void_p), ("g_lo", c_void_p)]
 
 
+class GlowFluid(ctypes.Structure):
+    _fields_ = [("n", c_int), ("DpT", c_void_p), ("DppT", c_void_p), ("GT", c_void_p), ("mu_i", c_double),
+                ("Di", c_double), ("n0", c_double), ("alpha", c_double), ("field_scale", c_double),
+                ("atol", c_double), ("rtol", c_double), ("max_iter", c_int), ("u", c_void_p), ("ki", c_void_p),
+                ("E", c_void_p), ("bsE", c_void_p), ("partial", c_void_p), ("nsplit", c_int),
+                ("rate_scale", c_double), ("status", c_void_p), ("info", c_void_p)]
+
+
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;
 # every entry point returns int
 _SIGS = {}

@@ -435,6 +435,61 @@ int bz_ee_jacobian_t2(int ns, int npr, int B, const double* ccA, const double* c
 int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double* top, const double* ne, double c_m0,
                 double c_kt, double c_lam, double c_gam, double* negam, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Plasma coupling of the 1D3V glow-discharge step (glowdischarge_boltzmann_1d.py).
+ * ------------------------------------------------------------------------------------ */
+
+/* Phases of bz_glow_fluid_step, run in this order when set. */
+#define BZ_GLOW_MOMENTS 1     /* bte_to_fluid (:1050-1076): ne, Te, ki from partial moments  */
+#define BZ_GLOW_STEP 2        /* field Dp(-phi) (:2221-2222) + backward-Euler ion step       */
+#define BZ_GLOW_FIELD_AFTER 4 /* fluid_to_bte (:1078-1086): bsE = (V0/L) Dp(-phi)            */
+
+/* Ion-fluid state and operators on n <= 200 Chebyshev points.  Matrices are stored
+ * transposed (MT[c n + r] = M[r][c]).  u is (n, 3) row-major: ne, ni, Te (the reference's
+ * Uin with Ns = 2, NT = 1).  status[0] converged (1) or not (0) for the last step,
+ * status[1] its Newton iterations, status[2] a failure count that only grows, status[3] a
+ * zero-pivot latch; info[0..2] the last |r|, |r|/|r0| and damping factor. */
+typedef struct {
+  int n;
+  const double* DpT;   /* Dp, the Chebyshev first-derivative matrix                     */
+  const double* DppT;  /* Dp Dp_mod, Dp_mod = Dp with rows 0 and n-1 set to zero        */
+  const double* GT;    /* -Dp LpD^-1: field of a Poisson right-hand side               */
+  double mu_i, Di;     /* ion mobility and diffusivity, nondimensional, uniform in x    */
+  double n0;           /* gas density / np0                                             */
+  double alpha;        /* np0 L^2 qe / (eps0 V0)                                        */
+  double field_scale;  /* V0 / L                                                        */
+  double atol, rtol;
+  int max_iter;
+  double* u;
+  double* ki;          /* (n) ionisation rate per RF period per np0                     */
+  double* E;           /* (n) nondimensional field of the last ion step                 */
+  double* bsE;         /* (n) field of the electron v-step, V/m                         */
+  const double* partial; /* (nsplit, 3, n) from bz_glow_moments                        */
+  int nsplit;
+  double rate_scale;   /* sqrt(4 pi) np0 tau / (2 (vth / c_gamma)^3)                    */
+  int* status;
+  double* info;
+} BzGlowFluid;
+
+/* Partial moments of the ordinate state: partial[sp][m][x] = sum over the rows of split sp
+ * (rows split in nsplit equal ranges) of W[m][row] v[row][x], v (rows, nx) with rows =
+ * num_p Nvt and W[m][(r, j)] = w_m[r] po2sh[0][j] for w = mass, temperature and ionisation
+ * rate operators: the l = 0 projection and the dot products of bte_to_fluid
+ * (glowdischarge_boltzmann_1d.py:1063-1072) in one pass over v. */
+int bz_glow_moments(const double* v, int rows, int nx, const double* W, int nsplit, double* partial,
+                    hipStream_t stream);
+
+/* One launch of the ion-fluid side of the Strang step (glowdischarge_boltzmann_1d.py:
+ * 2214-2241): MOMENTS finishes bte_to_fluid; STEP computes the frozen field from the Poisson
+ * problem with phi(-1) = bc_before and advances ni by the backward-Euler step of length h
+ * with the Newton iteration of glow1d_utils.newton_solver (glow1d_utils.py:131-207) on the
+ * residual and Jacobian of rhs_fluid / rhs_fluid_jacobian (:1088-1218, weak ion flux at both
+ * ends); FIELD_AFTER writes the v-step field with phi(-1) = bc_after.  Replaces the host
+ * orchestration of step_fluid (:1318-1372) and its numpy/cupy inverse.  One workgroup; the
+ * Jacobian inverse is Gauss-Jordan with partial pivoting in registers. */
+int bz_glow_fluid_step(const BzGlowFluid* g, int flags, double h, double bc_before, double bc_after,
+                       hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
