@@ -384,46 +384,85 @@ def bench_transient(args):
 
 
 def bench_glow(args):
-    """1D3V velocity-space step at the glow shapes: Nr = 128 (129 B-splines), lmax = 2,
-    Nvt = 32, Nx = 200 Chebyshev-Gauss-Lobatto points, E = 1e4 x^3 V/m (spatially varying, so
-    GMRES with restart 30 is used), dt = 5e-5 RF periods: one x-advection half step, one
-    backward-Euler v-step, one x-advection half step."""
+    """The coupled 1D3V glow-discharge step (glowdischarge_boltzmann_1d.glow1d_boltzmann):
+    Nx = 200 Chebyshev points on the 2.54 cm gap, Nr = 128 (129 B-splines), lmax = 2,
+    Nvt = 32 ordinates, spline_qpts = 11, the 100 V / 13.56 MHz sinusoidal electrode
+    potential, 3-process synthetic argon, dt = 5e-5 RF periods (the reference's parameter
+    file, params/glow1d_hybrid_base.toml).
+    Every step is the reference's Strang split: Poisson field + backward-Euler ion half step,
+    x half step, backward-Euler v-step (GMRES restart 30 with the E-binned block
+    preconditioner, rtol 1e-10), x half step, moments, ion half step.  The timed run
+    continues the state from the warm-up (no step is repeated).  cpu_baseline: the oracle's
+    step (numpy, dense per-point v solves) on the host, two steps."""
     import torch
-    from boltzmann_amd.glow1d import Ordinates, VStep, XAdvection
-    g, s, dev = _grid_and_solver(q=11)               # spline_qpts = 11 for the glow grid
-    Minv = torch.linalg.inv(s.M)
-    C_en, C_gT, A_v = Minv @ sum(s.C), Minv @ s.Cg, Minv @ s.A
-    tau, np0, Tg, dt = 1.0 / 13.56e6, 8e16, 300.0, 5e-5
-    n0 = 3.22e22 / np0
-    Nx, Nvt = 200, 32
-    xg = -np.cos(np.pi * np.arange(Nx) / (Nx - 1))
-    E = torch.from_numpy(1e4 * xg ** 3).to(dev)
-    ords = Ordinates(g.num_p, g.lmax, Nvt, dev)
-    vs = VStep(C_en, C_gT, A_v, tau, n0, np0, Tg, dt, ordinates=ords, rtol=1e-10, gmres_rsrt=30)
-    xa = XAdvection(g, xg, Nvt, g.vth * tau / 0.5 / 2.54e-2, 0.5 * dt)
-    v0 = ords.sph_to_ords(s.initial(Nx).contiguous())
-
-    def step(v):
-        v = xa.step(v, 0.5 * dt)
-        v = vs.step(v.contiguous(), E)
-        return xa.step(v, 0.5 * dt)
-    for _ in range(args.warmup):
-        step(v0)
-    torch.cuda.synchronize()
+    from boltzmann_amd.glowdischarge_boltzmann_1d import default_args, glow1d_boltzmann
+    dt = 5e-5
+    a = default_args(collisions=XSEC, l_max=2, Nr=128, Np=200, Nvt=32, Te=TE, ev_max=EVMAX, ev_extend=1,
+                     spline_qpts=11, cfl=dt, rtol=1e-10, atol=1e-20, gmres_rsrt=30, max_iter=50)
     t0 = time.perf_counter()
-    v = v0
-    for _ in range(args.steps):
-        v = step(v0)
+    glow = glow1d_boltzmann(a)
+    u, v = glow.initialize()
+    u, v = glow.step_init(u, v, dt)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    t = 0.0
+    for _ in range(args.warmup):
+        v = glow._step_inplace(u, v, t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        try:
+            v = glow._step_inplace(u, v, t, dt)
+        except RuntimeError:
+            print(f"glow step {k} at t = {t:.6g} failed: bs_E in [{float(glow.bs_E.min()):.4g}, "
+                  f"{float(glow.bs_E.max()):.4g}] V/m, ne in [{float(u[:, 0].min()):.4g}, {float(u[:, 0].max()):.4g}]",
+                  file=sys.stderr, flush=True)
+            raise
+        iters.append(glow._vstep.last_iters)
+        t += dt
     torch.cuda.synchronize()
     dt_w = time.perf_counter() - t0
+    glow.check_fluid_status()
+    cpu = None
+    if not args.no_cpu:
+        cpu = glow_cpu_baseline(glow, u, v, t, dt)
     print(json.dumps({
-        "metric": "1D3V glow step (x-half, v, x-half) per second", "value": args.steps / dt_w, "unit": "steps/s",
+        "metric": "1D3V glow-discharge coupled steps per second", "value": args.steps / dt_w, "unit": "steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt_w / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "1D3V step, Nr=128 lmax=2 Nvt=32 Nx=200 spline_qpts=11, E=1e4 x^3 V/m, "
-                               "GMRES restart 30",
-                   "dof_v": g.n, "Nx": Nx, "Nvt": Nvt},
-        "gmres_iterations": vs.last_iters}), flush=True)
+        "config": {"workload": "1D3V glow discharge, Nx=200 Chebyshev points, Nr=128 lmax=2 Nvt=32 spline_qpts=11, "
+                               "100 V 13.56 MHz, dt=5e-5 RF periods, Strang split with the ion fluid, "
+                               "GMRES restart 30 rtol 1e-10",
+                   "dof_v": glow.dof_v, "Nx": glow.Np, "Nvt": glow.Nvt},
+        "gmres_iterations_mean": float(np.mean(iters)), "gmres_iterations_max": int(np.max(iters)),
+        "rf_periods_simulated": t, "setup_s": setup_s, "cpu_baseline": cpu}), flush=True)
+
+
+def glow_cpu_baseline(glow, u, v, t, dt, nsteps=2):
+    """The oracle's coupled step (oracle/glow_coupled.py: numpy, one dense solve per point for
+    the v-step) from the GPU state, timed on the host."""
+    import os as _os
+    from oracle import glow_coupled as gc
+    from oracle import operators as op
+    from oracle import solver0d as s0
+    from oracle import xsec
+    recs, _ = xsec.read_cross_section_data(XSEC)
+    procs = op.processes_from_lxcat(recs)
+    g = op.grid_batched(128, 2, TE, EVMAX, 11, [p.threshold for p in procs])
+    ops = s0.BatchedOps(g, procs)
+    ion = [i for i, p in enumerate(procs) if p.type == "IONIZATION"][0]
+    G = gc.GlowCoupled(ops, g, procs, glow.Np, glow.Nvt, dt, gc.Parameters(), ion, 1e-20, 1e-10, 50)
+    uo, vo = u.cpu().numpy(), v.cpu().numpy()
+    G.ki = glow._ki.cpu().numpy()
+    G.step(uo, vo, t)                               # first call: operator set-up paths warm
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        uo, vo = G.step(uo, vo, t + k * dt)
+    el = time.perf_counter() - t0
+    return {"value": nsteps / el, "unit": "steps/s", "cores": len(_os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"{nsteps} coupled steps of the same configuration (numpy oracle, BLAS threads)"}
 
 
 def run_serving(N, rate):

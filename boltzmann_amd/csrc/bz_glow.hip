@@ -28,8 +28,8 @@ namespace {
 constexpr int GF_THREADS = 1024;
 constexpr int GF_WAVES = GF_THREADS / 64;
 constexpr int GF_NMAX = 200;
-constexpr int GF_SMAX = 40;        // columns per thread at Np = 200 (CG = 5)
 constexpr int GF_NPAD = 384;       // >= CG ceil(Np / CG) for every Np <= 200: padded vectors
+constexpr int GF_TILE_RED = 5000;  // >= max(TR, TC) Npad of the tile layouts below (25 x 200 at 8 x 10)
 
 enum { V_NE = 0, V_NI, V_E, V_X, V_RR, V_R0, V_W, V_Z, V_DX, V_COUNT };
 
@@ -43,12 +43,16 @@ struct alignas(16) GfShared {
   int candi[GF_WAVES];
   int perm[GF_NMAX];
   int qinv[GF_NMAX];
+  double red2[GF_TILE_RED];      // tile path: per thread-column partial sums (T x Npad)
+  double tprow[2][GF_NMAX + 8];  // tile path: pivot row / column, double-buffered by step
+  double tfcol[2][GF_NMAX + 8];
+  int flag;
   double sc[4];
   int isc[4];
 };
 
 struct GfCtx {
-  int n, CG, r, cg, t;
+  int n, CG, r, cg, t, nw;
   bool act;
 };
 
@@ -78,7 +82,7 @@ __device__ __forceinline__ double gf_norm(const GfCtx& q, GfShared& S, const dou
   __syncthreads();
   if (q.t == 0) {
     double s = 0.0;
-    for (int w = 0; w < GF_WAVES; ++w) s += S.wsum[w];
+    for (int w = 0; w < q.nw; ++w) s += S.wsum[w];
     S.sc[0] = sqrt(s);
   }
   __syncthreads();
@@ -253,20 +257,229 @@ __device__ __forceinline__ int gf_gauss_jordan(const GfCtx& q, GfShared& S, doub
   return singular;
 }
 
-template <int SMAX>
-__global__ __launch_bounds__(GF_THREADS) void k_glow_fluid(BzGlowFluid g, int flags, double h, double bc_before,
+// Newton with the frozen inverse and the damping restarts of glow1d_utils.newton_solver
+// (glow1d_utils.py:131-207): apply(rr, dx) writes dx = J^-1 rr.  Writes ni and the status.
+template <typename Apply>
+__device__ __forceinline__ void gf_newton(const GfCtx& q, GfShared& S, const BzGlowFluid& g, double h, int singular,
+                                          Apply apply) {
+  const int n = q.n, t = q.t;
+  double* x = S.vec[V_X];
+  double* rr = S.vec[V_RR];
+  double* r0 = S.vec[V_R0];
+  if (t < n) x[t] = 0.0;
+  __syncthreads();
+  gf_residual(q, S, g, h, x, r0);
+  const double nr0 = gf_norm(q, S, r0);
+  double nr = nr0;
+  bool conv = (nr / nr0 < g.rtol) || (nr < g.atol);
+  double alpha = 1.0;
+  int count = 0;
+  while (alpha > 1e-6) {
+    count = 0;
+    if (t < n) {
+      x[t] = 0.0;
+      rr[t] = r0[t];
+    }
+    __syncthreads();
+    while (!conv && count < g.max_iter) {
+      apply(rr, S.vec[V_DX]);
+      if (t < n) x[t] = x[t] - alpha * S.vec[V_DX][t];
+      __syncthreads();
+      gf_residual(q, S, g, h, x, rr);
+      nr = gf_norm(q, S, rr);
+      ++count;
+      conv = (nr / nr0 < g.rtol) || (nr < g.atol);
+      if (isnan(nr) || nr > nr0 * 1e2) break;
+    }
+    if (!conv) alpha *= 0.25;
+    else break;
+  }
+  if (t < n && conv) {
+    const double ni1 = S.vec[V_NI][t] + x[t];
+    S.vec[V_NI][t] = ni1;
+    g.u[3 * t + 1] = ni1;
+  }
+  if (t == 0) {
+    g.status[0] = conv ? 1 : 0;
+    g.status[1] = count;
+    if (!conv) g.status[2] += 1;
+    g.status[3] = (g.status[3] & 1) | singular;   // bit 0 latches a zero pivot, bit 1 the last path
+    g.info[0] = nr;
+    g.info[1] = nr / nr0;
+    g.info[2] = alpha;
+  }
+  __syncthreads();
+}
+
+// Tile layout of the Jacobian for the pivot-free path: thread (tr, tc), tr < TR, tc < TC,
+// holds the BR x BC tile of rows tr BR + i and columns tc BC + j of the matrix padded with
+// identity rows and columns to Npad, the multiple of lcm(BR, BC) at or above n.
+constexpr int gf_gcd(int a, int b) { return b == 0 ? a : gf_gcd(b, a % b); }
+
+struct GfTile {
+  int TR, TC, tr, tc, Npad;
+  bool act;
+};
+
+template <int BR, int BC>
+__device__ __forceinline__ GfTile gf_tile_ctx(const GfCtx& q) {
+  constexpr int L = BR / gf_gcd(BR, BC) * BC;      // lcm: both rotations are back in place
+  GfTile tl;
+  tl.Npad = ((q.n + L - 1) / L) * L;
+  tl.TR = tl.Npad / BR;
+  tl.TC = tl.Npad / BC;
+  tl.act = q.t < tl.TR * tl.TC;
+  tl.tr = tl.act ? q.t / tl.TC : 0;
+  tl.tc = tl.act ? q.t % tl.TC : 0;
+  return tl;
+}
+
+// J[r][c] = delta_rc + h (mu_i Dp[r][c] E[c] - Di (Dp Dp_mod)[r][c]) into the tiles, and
+// whether J is column diagonally dominant (|J_cc| > sum_{r != c} |J_rc| for every c).  For
+// such a matrix partial pivoting never exchanges rows, so elimination in the natural order
+// takes the same pivots as the reference's LAPACK inverse.
+template <int BR, int BC>
+__device__ __forceinline__ bool gf_tile_build(const GfCtx& q, GfShared& S, const BzGlowFluid& g, double h,
+                                              const GfTile& tl, double (&X)[BR][BC]) {
+  const int n = q.n;
+#pragma unroll
+  for (int i = 0; i < BR; ++i)
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+      const int r = tl.tr * BR + i, c = tl.tc * BC + j;
+      const bool in = tl.act && r < n && c < n;
+      const size_t o = (size_t)(in ? c : 0) * n + (in ? r : 0);
+      const double v = (r == c ? 1.0 : 0.0) + h * (g.mu_i * g.DpT[o] * S.vec[V_E][in ? c : 0] - g.Di * g.DppT[o]);
+      X[i][j] = in ? v : (r == c ? 1.0 : 0.0);
+    }
+  // off-diagonal column sums per thread row, then per column
+  if (tl.act) {
+#pragma unroll
+    for (int j = 0; j < BC; ++j) {
+      const int c = tl.tc * BC + j;
+      double sabs = 0.0;
+#pragma unroll
+      for (int i = 0; i < BR; ++i) {
+        const int r = tl.tr * BR + i;
+        if (r != c) sabs += fabs(X[i][j]);
+        else S.tprow[0][c] = fabs(X[i][j]);
+      }
+      S.red2[tl.tr * tl.Npad + c] = sabs;
+    }
+  }
+  if (q.t == 0) S.flag = 1;
+  __syncthreads();
+  if (q.t < tl.Npad) {
+    double sum = 0.0;
+    for (int r = 0; r < tl.TR; ++r) sum += S.red2[r * tl.Npad + q.t];
+    if (!(S.tprow[0][q.t] > sum)) S.flag = 0;
+  }
+  __syncthreads();
+  const bool dom = S.flag != 0;
+  __syncthreads();
+  return dom;
+}
+
+// In-place Gauss-Jordan inverse without row exchanges on the tiles.  All threads rotate their
+// tiles by one row (period BR) and one column (period BC) every step, so the pivot row of
+// step k is tile row 0 of thread row k / BR and the pivot column tile column 0 of thread
+// column k / BC: the loop body uses constant register indices, the pivot row and column go
+// to LDS as BC / BR values per thread (indexed in the rotated frame all threads share), one
+// barrier a step.  After Npad steps every tile is back in place.
+template <int BR, int BC>
+__device__ __forceinline__ int gf_tile_gauss_jordan(const GfCtx& q, GfShared& S, const GfTile& tl,
+                                                    double (&X)[BR][BC]) {
+  int singular = 0;
+  if (tl.act && tl.tr == 0)
+#pragma unroll
+    for (int j = 0; j < BC; ++j) S.tprow[0][tl.tc * BC + j] = X[0][j];
+  if (tl.act && tl.tc == 0)
+#pragma unroll
+    for (int i = 0; i < BR; ++i) S.tfcol[0][tl.tr * BR + i] = X[i][0];
+  __syncthreads();
+  for (int k = 0; k < tl.Npad; ++k) {
+    const int dr = k / BR, dc = k / BC, buf = k & 1;
+    const double piv = S.tprow[buf][dc * BC];
+    const double inv = 1.0 / piv;
+    if (!(piv != 0.0)) singular = 1;
+    if (tl.act) {
+      const bool pr = tl.tr == dr, pc = tl.tc == dc;
+      double qv[BC];
+#pragma unroll
+      for (int j = 0; j < BC; ++j) qv[j] = S.tprow[buf][tl.tc * BC + j] * inv;
+      if (pc) qv[0] = inv;
+      const double* fc = &S.tfcol[buf][tl.tr * BR];
+      double row0[BC];
+#pragma unroll
+      for (int j = 0; j < BC; ++j) row0[j] = X[0][j];
+      // new(i, j) = base(i, j) - f_i q_j (the pivot row takes q: f = -1 on a zero base; the
+      // pivot column -f / piv: zero base), stored one row and one column down; in program
+      // order X[i][j] is written after X[i+1][j+1] is read, except the wrap to row 0
+#pragma unroll
+      for (int i = 0; i < BR; ++i) {
+        const int si = (i + 1) % BR;
+        const double fi = (pr && si == 0) ? -1.0 : fc[si];
+#pragma unroll
+        for (int j = 0; j < BC; ++j) {
+          const int sj = (j + 1) % BC;
+          double base = (si == 0) ? row0[sj] : X[si][sj];
+          if ((pr && si == 0) || (pc && sj == 0)) base = 0.0;
+          X[i][j] = fma(-fi, qv[sj], base);
+        }
+      }
+    }
+    const int k1 = k + 1;
+    if (k1 < tl.Npad && tl.act) {
+      const int nb = buf ^ 1;
+      if (tl.tr == k1 / BR)
+#pragma unroll
+        for (int j = 0; j < BC; ++j) S.tprow[nb][tl.tc * BC + j] = X[0][j];
+      if (tl.tc == k1 / BC)
+#pragma unroll
+        for (int i = 0; i < BR; ++i) S.tfcol[nb][tl.tr * BR + i] = X[i][0];
+    }
+    __syncthreads();
+  }
+  return singular;
+}
+
+// dx = X rr for the tile-resident inverse (no permutation on this path).
+template <int BR, int BC>
+__device__ __forceinline__ void gf_tile_apply(const GfCtx& q, GfShared& S, const GfTile& tl,
+                                              const double (&X)[BR][BC], const double* rr, double* dx) {
+  if (tl.act) {
+#pragma unroll
+    for (int i = 0; i < BR; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < BC; ++j) acc = fma(X[i][j], rr[tl.tc * BC + j], acc);
+      S.red2[tl.tc * tl.Npad + tl.tr * BR + i] = acc;
+    }
+  }
+  __syncthreads();
+  if (q.t < q.n) {
+    double s = 0.0;
+    for (int c = 0; c < tl.TC; ++c) s += S.red2[c * tl.Npad + q.t];
+    dx[q.t] = s;
+  }
+  __syncthreads();
+}
+
+template <int NT, int SMAX, int BR, int BC>
+__global__ __launch_bounds__(NT) void k_glow_fluid(BzGlowFluid g, int flags, double h, double bc_before,
                                                            double bc_after) {
   __shared__ GfShared S;
   GfCtx q;
   q.n = g.n;
   q.t = threadIdx.x;
-  q.CG = GF_THREADS / q.n;
+  q.CG = NT / q.n;
+  q.nw = NT / 64;
   q.r = q.t % q.n;
   q.cg = q.t / q.n;
   q.act = q.cg < q.CG;
   const int n = q.n, t = q.t;
 
-  for (int i = t; i < V_COUNT * GF_NPAD; i += GF_THREADS) (&S.vec[0][0])[i] = 0.0;
+  for (int i = t; i < V_COUNT * GF_NPAD; i += NT) (&S.vec[0][0])[i] = 0.0;
   __syncthreads();
   if (t < n) {
     S.vec[V_NE][t] = g.u[3 * t + 0];
@@ -303,70 +516,38 @@ __global__ __launch_bounds__(GF_THREADS) void k_glow_fluid(BzGlowFluid g, int fl
     gf_field(q, S, g, bc_before, S.vec[V_E]);
     if (t < n) g.E[t] = S.vec[V_E][t];
 
-    // J[r][c] = delta_rc + h (mu_i Dp[r][c] E[c] - Di (Dp Dp_mod)[r][c])
-    double X[SMAX];
-    const int Sc = (n + q.CG - 1) / q.CG;
+    bool done = false;
+    if constexpr (BR > 0) {
+      // pivot-free tile path when J is column diagonally dominant
+      const GfTile tl = gf_tile_ctx<BR, BC>(q);
+      double Xt[BR][BC];
+      if (gf_tile_build<BR, BC>(q, S, g, h, tl, Xt)) {
+        const int singular = gf_tile_gauss_jordan<BR, BC>(q, S, tl, Xt);
+        gf_newton(q, S, g, h, singular,
+                  [&](const double* rr, double* dx) { gf_tile_apply<BR, BC>(q, S, tl, Xt, rr, dx); });
+        done = true;
+      }
+    }
+    if (!done) {
+      // J[r][c] = delta_rc + h (mu_i Dp[r][c] E[c] - Di (Dp Dp_mod)[r][c]), row layout,
+      // Gauss-Jordan with partial pivoting
+      double X[SMAX];
+      const int Sc = (n + q.CG - 1) / q.CG;
 #pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      X[s] = 0.0;
-      if (s < Sc) {
-        const int c = q.cg * Sc + s;
-        const bool in = q.act && c < n;
-        const size_t o = (size_t)(in ? c : 0) * n + q.r;     // padding lanes read a valid entry
-        const double v = (q.r == c ? 1.0 : 0.0) + h * (g.mu_i * g.DpT[o] * S.vec[V_E][c] - g.Di * g.DppT[o]);
-        X[s] = in ? v : 0.0;
+      for (int s = 0; s < SMAX; ++s) {
+        X[s] = 0.0;
+        if (s < Sc) {
+          const int c = q.cg * Sc + s;
+          const bool in = q.act && c < n;
+          const size_t o = (size_t)(in ? c : 0) * n + q.r;     // padding lanes read a valid entry
+          const double v = (q.r == c ? 1.0 : 0.0) + h * (g.mu_i * g.DpT[o] * S.vec[V_E][c] - g.Di * g.DppT[o]);
+          X[s] = in ? v : 0.0;
+        }
       }
+      const int singular = gf_gauss_jordan<SMAX>(q, S, X, Sc) | 2;     // bit 1: pivoted path
+      gf_newton(q, S, g, h, singular,
+                [&](const double* rr, double* dx) { gf_apply_inverse<SMAX>(q, S, X, Sc, rr, dx); });
     }
-
-    const int singular = gf_gauss_jordan<SMAX>(q, S, X, Sc);
-
-    // Newton with the frozen inverse and the damping restarts of the reference
-    double* x = S.vec[V_X];
-    double* rr = S.vec[V_RR];
-    double* r0 = S.vec[V_R0];
-    if (t < n) x[t] = 0.0;
-    __syncthreads();
-    gf_residual(q, S, g, h, x, r0);
-    const double nr0 = gf_norm(q, S, r0);
-    double nr = nr0;
-    bool conv = (nr / nr0 < g.rtol) || (nr < g.atol);
-    double alpha = 1.0;
-    int count = 0;
-    while (alpha > 1e-6) {
-      count = 0;
-      if (t < n) {
-        x[t] = 0.0;
-        rr[t] = r0[t];
-      }
-      __syncthreads();
-      while (!conv && count < g.max_iter) {
-        gf_apply_inverse<SMAX>(q, S, X, Sc, rr, S.vec[V_DX]);
-        if (t < n) x[t] = x[t] - alpha * S.vec[V_DX][t];
-        __syncthreads();
-        gf_residual(q, S, g, h, x, rr);
-        nr = gf_norm(q, S, rr);
-        ++count;
-        conv = (nr / nr0 < g.rtol) || (nr < g.atol);
-        if (isnan(nr) || nr > nr0 * 1e2) break;
-      }
-      if (!conv) alpha *= 0.25;
-      else break;
-    }
-    if (t < n && conv) {
-      const double ni1 = S.vec[V_NI][t] + x[t];
-      S.vec[V_NI][t] = ni1;
-      g.u[3 * t + 1] = ni1;
-    }
-    if (t == 0) {
-      g.status[0] = conv ? 1 : 0;
-      g.status[1] = count;
-      if (!conv) g.status[2] += 1;
-      g.status[3] |= singular;
-      g.info[0] = nr;
-      g.info[1] = nr / nr0;
-      g.info[2] = alpha;
-    }
-    __syncthreads();
   }
 
   if (flags & BZ_GLOW_FIELD_AFTER) {
@@ -425,15 +606,23 @@ extern "C" int bz_glow_fluid_step(const BzGlowFluid* g, int flags, double h, dou
                                  g->max_iter < 0))
     return BZ_EINVAL;
   if ((flags & BZ_GLOW_FIELD_AFTER) && (!g->GT || !g->bsE)) return BZ_EINVAL;
+  // tile layouts: 8 x 10 for 128 < n <= 200 (Npad <= 200, 25 x 20 threads), 6 x 6 for
+  // 90 < n <= 128 (Npad <= 132, 22 x 22 threads): LDS reductions of max(TR, TC) Npad entries
+  static_assert(25 * 200 <= GF_TILE_RED && 22 * 132 <= GF_TILE_RED, "tile reduction buffer");
+  static_assert(25 * 20 <= 512 && 22 * 22 <= 512, "tile threads");
   const BzGlowFluid gv = *g;
+  // register layout: the matrix fits in registers either at 1024 threads (4 waves per SIMD,
+  // 128 registers each) with up to 8 columns per thread, or at 512 threads (256 registers)
+  // with up to 32 (n <= 128) or 100 columns per thread
   const int CG = GF_THREADS / g->n, Sc = (g->n + CG - 1) / CG;
   if (Sc <= 2)
-    hipLaunchKernelGGL(k_glow_fluid<2>, dim3(1), dim3(GF_THREADS), 0, stream, gv, flags, h, bc_before, bc_after);
+    hipLaunchKernelGGL((k_glow_fluid<1024, 2, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
   else if (Sc <= 8)
-    hipLaunchKernelGGL(k_glow_fluid<8>, dim3(1), dim3(GF_THREADS), 0, stream, gv, flags, h, bc_before, bc_after);
+    hipLaunchKernelGGL((k_glow_fluid<1024, 8, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
+  else if (g->n <= 128)
+    hipLaunchKernelGGL((k_glow_fluid<512, 32, 6, 6>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before, bc_after);
   else
-    hipLaunchKernelGGL(k_glow_fluid<GF_SMAX>, dim3(1), dim3(GF_THREADS), 0, stream, gv, flags, h, bc_before,
-                       bc_after);
+    hipLaunchKernelGGL((k_glow_fluid<512, 100, 8, 10>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before, bc_after);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

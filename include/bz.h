@@ -447,8 +447,9 @@ int bz_ee_gamma(const double* x, int n, int B, const double* mass, const double*
 /* Ion-fluid state and operators on n <= 200 Chebyshev points.  Matrices are stored
  * transposed (MT[c n + r] = M[r][c]).  u is (n, 3) row-major: ne, ni, Te (the reference's
  * Uin with Ns = 2, NT = 1).  status[0] converged (1) or not (0) for the last step,
- * status[1] its Newton iterations, status[2] a failure count that only grows, status[3] a
- * zero-pivot latch; info[0..2] the last |r|, |r|/|r0| and damping factor. */
+ * status[1] its Newton iterations, status[2] a failure count that only grows, status[3] bit 0
+ * a zero-pivot latch and bit 1 set when the last step took the partially pivoted inverse (J
+ * not column diagonally dominant); info[0..2] the last |r|, |r|/|r0| and damping factor. */
 typedef struct {
   int n;
   const double* DpT;   /* Dp, the Chebyshev first-derivative matrix                     */
