@@ -50,7 +50,8 @@ class GlowFluid(ctypes.Structure):
                 ("Di", c_double), ("n0", c_double), ("alpha", c_double), ("field_scale", c_double),
                 ("atol", c_double), ("rtol", c_double), ("max_iter", c_int), ("u", c_void_p), ("ki", c_void_p),
                 ("E", c_void_p), ("bsE", c_void_p), ("partial", c_void_p), ("nsplit", c_int),
-                ("rate_scale", c_double), ("status", c_void_p), ("info", c_void_p)]
+                ("rate_scale", c_double), ("status", c_void_p), ("info", c_void_p), ("DpTile", c_void_p),
+                ("DppTile", c_void_p)]
 
 
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;

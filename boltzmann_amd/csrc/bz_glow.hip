@@ -29,6 +29,7 @@ constexpr int GF_THREADS = 1024;
 constexpr int GF_WAVES = GF_THREADS / 64;
 constexpr int GF_NMAX = 200;
 constexpr int GF_NPAD = 384;       // >= CG ceil(Np / CG) for every Np <= 200: padded vectors
+constexpr int BZ_GLOW_TILE_THREADS = 512;   // block size of the tile layouts
 constexpr int GF_TILE_RED = 5000;  // >= max(TR, TC) Npad of the tile layouts below (25 x 200 at 8 x 10)
 
 enum { V_NE = 0, V_NI, V_E, V_X, V_RR, V_R0, V_W, V_Z, V_DX, V_COUNT };
@@ -348,8 +349,17 @@ __device__ __forceinline__ bool gf_tile_build(const GfCtx& q, GfShared& S, const
     for (int j = 0; j < BC; ++j) {
       const int r = tl.tr * BR + i, c = tl.tc * BC + j;
       const bool in = tl.act && r < n && c < n;
-      const size_t o = (size_t)(in ? c : 0) * n + (in ? r : 0);
-      const double v = (r == c ? 1.0 : 0.0) + h * (g.mu_i * g.DpT[o] * S.vec[V_E][in ? c : 0] - g.Di * g.DppT[o]);
+      double dp, dpp;
+      if (g.DpTile) {                               // tile-ordered copies: coalesced
+        const size_t o = (size_t)(i * BC + j) * BZ_GLOW_TILE_THREADS + q.t;
+        dp = g.DpTile[o];
+        dpp = g.DppTile[o];
+      } else {
+        const size_t o = (size_t)(in ? c : 0) * n + (in ? r : 0);
+        dp = g.DpT[o];
+        dpp = g.DppT[o];
+      }
+      const double v = (r == c ? 1.0 : 0.0) + h * (g.mu_i * dp * S.vec[V_E][in ? c : 0] - g.Di * dpp);
       X[i][j] = in ? v : (r == c ? 1.0 : 0.0);
     }
   // off-diagonal column sums per thread row, then per column
@@ -588,6 +598,54 @@ __global__ __launch_bounds__(256) void k_glow_moments(const double* __restrict__
 
 }  // namespace
 
+// The kernel variant (register layout) bz_glow_fluid_step uses for n points: 0, 1: row
+// layouts of 2 and 8 columns per thread at 1024 threads; 2: 6 x 6 tiles at 512; 3: 8 x 10.
+static int gf_variant(int n) {
+  const int CG = GF_THREADS / n, Sc = (n + CG - 1) / CG;
+  if (Sc <= 2) return 0;
+  if (Sc <= 8) return 1;
+  return n <= 128 ? 2 : 3;
+}
+
+template <int BR, int BC>
+__global__ __launch_bounds__(256) void k_glow_tile_order(const double* __restrict__ DpT,
+                                                         const double* __restrict__ DppT, int n,
+                                                         double* __restrict__ DpTile, double* __restrict__ DppTile) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= BR * BC * BZ_GLOW_TILE_THREADS) return;
+  const int t = idx % BZ_GLOW_TILE_THREADS, e = idx / BZ_GLOW_TILE_THREADS;
+  const int i = e / BC, j = e % BC;
+  constexpr int L = BR / gf_gcd(BR, BC) * BC;
+  const int Npad = ((n + L - 1) / L) * L, TC = Npad / BC, TR = Npad / BR;
+  double a = 0.0, b = 0.0;
+  if (t < TR * TC) {
+    const int r = (t / TC) * BR + i, c = (t % TC) * BC + j;
+    if (r < n && c < n) {
+      a = DpT[(size_t)c * n + r];
+      b = DppT[(size_t)c * n + r];
+    }
+  }
+  DpTile[idx] = a;
+  DppTile[idx] = b;
+}
+
+extern "C" int bz_glow_tile_tables(int n, const double* DpT, const double* DppT, double* DpTile, double* DppTile,
+                                   hipStream_t stream) {
+  if (n < 3 || n > GF_NMAX || !DpT || !DppT || !DpTile || !DppTile) return BZ_EINVAL;
+  const int v = gf_variant(n);
+  if (v == 2) {
+    hipLaunchKernelGGL((k_glow_tile_order<6, 6>), dim3((36 * BZ_GLOW_TILE_THREADS + 255) / 256), dim3(256), 0,
+                       stream, DpT, DppT, n, DpTile, DppTile);
+  } else if (v == 3) {
+    hipLaunchKernelGGL((k_glow_tile_order<8, 10>), dim3((80 * BZ_GLOW_TILE_THREADS + 255) / 256), dim3(256), 0,
+                       stream, DpT, DppT, n, DpTile, DppTile);
+  } else {
+    return BZ_OK;
+  }
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
 extern "C" int bz_glow_moments(const double* v, int rows, int nx, const double* W, int nsplit, double* partial,
                                hipStream_t stream) {
   if (!v || !W || !partial || rows <= 0 || nx <= 0 || nsplit <= 0 || nsplit > 65535) return BZ_EINVAL;
@@ -614,15 +672,20 @@ extern "C" int bz_glow_fluid_step(const BzGlowFluid* g, int flags, double h, dou
   // register layout: the matrix fits in registers either at 1024 threads (4 waves per SIMD,
   // 128 registers each) with up to 8 columns per thread, or at 512 threads (256 registers)
   // with up to 32 (n <= 128) or 100 columns per thread
-  const int CG = GF_THREADS / g->n, Sc = (g->n + CG - 1) / CG;
-  if (Sc <= 2)
-    hipLaunchKernelGGL((k_glow_fluid<1024, 2, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
-  else if (Sc <= 8)
-    hipLaunchKernelGGL((k_glow_fluid<1024, 8, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
-  else if (g->n <= 128)
-    hipLaunchKernelGGL((k_glow_fluid<512, 32, 6, 6>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before, bc_after);
-  else
-    hipLaunchKernelGGL((k_glow_fluid<512, 100, 8, 10>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before, bc_after);
+  switch (gf_variant(g->n)) {
+    case 0:
+      hipLaunchKernelGGL((k_glow_fluid<1024, 2, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_glow_fluid<1024, 8, 0, 0>), dim3(1), dim3(1024), 0, stream, gv, flags, h, bc_before, bc_after);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_glow_fluid<512, 32, 6, 6>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before, bc_after);
+      break;
+    default:
+      hipLaunchKernelGGL((k_glow_fluid<512, 100, 8, 10>), dim3(1), dim3(512), 0, stream, gv, flags, h, bc_before,
+                         bc_after);
+  }
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

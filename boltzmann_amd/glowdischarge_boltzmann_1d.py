@@ -42,6 +42,7 @@ from .glow1d_utils import parameters
 from .vspace import C_GAMMA, QE, ME, TEMP_EV_FAC, VGrid, make_knots, vth_from_Te
 
 MOMENT_SPLITS = 64
+TILE_LEN = 512 * 80           # BZ_GLOW_TILE_LEN
 
 
 def args_parse(argv=None):
@@ -233,6 +234,10 @@ class glow1d_boltzmann:
         self._DpT = d(self.Dp.T)
         self._DppT = d((self.Dp @ Dp_mod).T)
         self._GT = d(G.T)
+        # the two matrices again in the order of the step kernel's register tiles
+        self._DpTile = torch.zeros(TILE_LEN, dtype=torch.float64, device=self.dev)
+        self._DppTile = torch.zeros(TILE_LEN, dtype=torch.float64, device=self.dev)
+        _lib.call("bz_glow_tile_tables", n, self._DpT, self._DppT, self._DpTile, self._DppTile)
         self._ki = torch.zeros(n, dtype=torch.float64, device=self.dev)
         self._Ef = torch.zeros(n, dtype=torch.float64, device=self.dev)
         self.bs_E = torch.zeros(n, dtype=torch.float64, device=self.dev)
@@ -248,7 +253,8 @@ class glow1d_boltzmann:
                                max_iter=int(self.args.max_iter), u=0, ki=self._ki.data_ptr(),
                                E=self._Ef.data_ptr(), bsE=self.bs_E.data_ptr(), partial=self._partial.data_ptr(),
                                nsplit=MOMENT_SPLITS, rate_scale=self.mm_fac * p.np0 * p.tau / c3,
-                               status=self._status.data_ptr(), info=self._info.data_ptr())
+                               status=self._status.data_ptr(), info=self._info.data_ptr(),
+                               DpTile=self._DpTile.data_ptr(), DppTile=self._DppTile.data_ptr())
         self.mu[:, self.ele_idx] = p._mu_e
         self.D[:, self.ele_idx] = p._De
         self.mu[:, self.ion_idx] = p.mu_i
@@ -313,12 +319,13 @@ class glow1d_boltzmann:
         g = self.vgrid
         vth = self.bs_vth
         Te = U[:, self.Te_idx]
-        cols = []
+        cols, cache = [], {}
         for i in range(self.Np):
             v_ratio = C_GAMMA * math.sqrt(Te[i]) / vth
-            h = self._maxwellian(v_ratio)
-            h = h / torch.dot(self.op_mass, h)
-            cols.append(h)
+            if v_ratio not in cache:
+                h = self._maxwellian(v_ratio)
+                cache[v_ratio] = h / torch.dot(self.op_mass, h)
+            cols.append(cache[v_ratio])
         H = torch.stack(cols, dim=1).contiguous()                      # (dof_v, Np)
         V = self.ords.sph_to_ords(H) * self._d(U[:, self.ele_idx])[None, :]
         return self._d(U), V.contiguous()

@@ -470,7 +470,12 @@ typedef struct {
   double rate_scale;   /* sqrt(4 pi) np0 tau / (2 (vth / c_gamma)^3)                    */
   int* status;
   double* info;
+  const double* DpTile;  /* optional (may be NULL): Dp and Dp Dp_mod in the order of the   */
+  const double* DppTile; /* kernel's register tiles, from bz_glow_tile_tables              */
 } BzGlowFluid;
+
+/* Length (doubles) of each table written by bz_glow_tile_tables. */
+#define BZ_GLOW_TILE_LEN (512 * 80)
 
 /* Partial moments of the ordinate state: partial[sp][m][x] = sum over the rows of split sp
  * (rows split in nsplit equal ranges) of W[m][row] v[row][x], v (rows, nx) with rows =
@@ -490,6 +495,13 @@ int bz_glow_moments(const double* v, int rows, int nx, const double* W, int nspl
  * Jacobian inverse is Gauss-Jordan with partial pivoting in registers. */
 int bz_glow_fluid_step(const BzGlowFluid* g, int flags, double h, double bc_before, double bc_after,
                        hipStream_t stream);
+
+/* Copies of Dp and Dp Dp_mod (given transposed, as in BzGlowFluid) in the order in which
+ * bz_glow_fluid_step's register tiles read them for n points (90 < n <= 200), so that the
+ * Jacobian is loaded with coalesced reads; each output holds BZ_GLOW_TILE_LEN doubles.  For
+ * n <= 90 the step keeps its row layout and nothing is written.  Set-up, once per n. */
+int bz_glow_tile_tables(int n, const double* DpT, const double* DppT, double* DpTile, double* DppTile,
+                        hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,14 @@ def test_fluid_launch_against_oracle(Np):
             assert _rel(glow.bs_E, bsE) < tol_E
         np.testing.assert_array_equal(ud[:, 0].cpu().numpy(), ne)
         np.testing.assert_array_equal(ud[:, 2].cpu().numpy(), 3.0)
+        # without the tile-ordered tables (a C caller may pass none): the same bits
+        tiles = glow._fl.DpTile, glow._fl.DppTile
+        glow._fl.DpTile = glow._fl.DppTile = None
+        u2 = torch.from_numpy(u.copy()).cuda()
+        glow._fluid_launch(u2, GLOW_STEP | GLOW_FIELD_AFTER, h=h, t_before=t0, t_after=t1)
+        torch.cuda.synchronize()
+        glow._fl.DpTile, glow._fl.DppTile = tiles
+        assert torch.equal(u2, ud)
 
 
 def test_fluid_failure_is_reported():
