@@ -465,6 +465,90 @@ def glow_cpu_baseline(glow, u, v, t, dt, nsteps=2):
             "sample": f"{nsteps} coupled steps of the same configuration (numpy oracle, BLAS threads)"}
 
 
+FVM_TABLE = dict(L=2.54e-2, V0=100.0, freq=13.56e6, Tg=300.0, p0=1.0, sp_order=3, spline_qpts=11, Nr=63, Np=100,
+                 Nvt=16, dt=2e-4, Te=TE, ev_max=EVMAX, ev_extend=2, atol=1e-12, rtol=1e-10, gmres_rsrt=5,
+                 gmres_rsrt_iter=10, pvr=3, pvt=3, px=1, solver_type="fully_implicit", vspace_type=0, xspace_type=0)
+
+
+def bench_fvm(args):
+    """The fully implicit 1D3V step on the finite-volume / ordinate grid
+    (bte_1d3v_solver.bte_1d3v, FULLY_IMPLICIT with the XVTVRC preconditioner): the sizes of the
+    reference's params/bte1d3v_base.toml (Np = 100 Chebyshev points, Nr = 63 B-splines with 11
+    points per knot interval = 671 radial points, log-extended grid, Nvt = 16 ordinates),
+    dt = 2e-4 RF periods, GMRES restart 5, at most 10 cycles, rtol 1e-10, in the reference
+    driver's field E(x, t) = 1e4 x^3 sin(2 pi t) V/m from t = 0.25 (its ef_mode 1 and the start
+    of its time-convergence run).  The timed steps continue the state from the warm-up.
+    cpu_baseline: one GMRES iteration (step operator + XVTVRC preconditioner) of the oracle at
+    the same size, times the GPU run's mean iteration count."""
+    import types
+    import torch
+    from boltzmann_amd.bte_1d3v_solver import bte_1d3v, params
+    tp = dict(FVM_TABLE, collisions=XSEC)
+    dt = tp["dt"]
+    t0 = time.perf_counter()
+    S = bte_1d3v(types.SimpleNamespace(params=params(tp=tp)))
+    S.init(dt)
+    v = S.initial_condition()
+    xp = torch.from_numpy(S.xp).cuda()
+    Ef = lambda t: 1e4 * xp ** 3 * math.sin(2 * math.pi * t)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    t = 0.25
+    for _ in range(args.warmup):
+        v = S.step(Ef(t), v, None, t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        v = S.step(Ef(t), v, None, t, dt)
+        iters.append(S.gmres_info["gmres_callback"].niter)
+        t += dt
+    torch.cuda.synchronize()
+    dt_w = time.perf_counter() - t0
+    q = S.compute_qoi(v)
+    cpu = None
+    if not args.no_cpu:
+        cpu = fvm_cpu_baseline(tp, S, v, Ef(t).cpu().numpy(), dt, float(np.mean(iters)))
+    print(json.dumps({
+        "metric": "1D3V fully implicit FVM steps per second", "value": args.steps / dt_w, "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt_w / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "1D3V fully implicit FVM step, XVTVRC-preconditioned GMRES (restart 5, 10 cycles, "
+                               "rtol 1e-10), Np=100 Chebyshev points, 671 radial x 16 ordinates, dt=2e-4 RF periods, "
+                               "E = 1e4 x^3 sin(2 pi t) V/m",
+                   "dof": S.dof_v * S.dof_x, "Nr": S.dof_vr, "Nvt": S.dof_vt, "Nx": S.dof_x},
+        "gmres_iterations_mean": float(np.mean(iters)), "gmres_iterations_max": int(np.max(iters)),
+        "ne_mean_m3": float(q["ne"].mean()), "Te_mean_eV": float(q["Te"].mean()),
+        "rf_periods_simulated": t - 0.25, "setup_s": setup_s, "cpu_baseline": cpu}), flush=True)
+
+
+def fvm_cpu_baseline(tp, S, v, E, dt, iters):
+    """One GMRES iteration of the oracle (oracle/fvm1d3v.py: dense operators, numpy with BLAS
+    threads) at the full size: the step operator and the XVTVRC preconditioner on one vector,
+    timed on the host and scaled by the GPU run's iterations per step."""
+    import os as _os
+    from oracle import fvm1d3v as fv
+    from oracle import operators as op
+    from oracle import xsec
+    recs, _ = xsec.read_cross_section_data(XSEC)
+    procs = op.processes_from_lxcat(recs)
+    keys = ("L", "V0", "freq", "Tg", "p0", "Np", "Nr", "Nvt", "sp_order", "spline_qpts", "ev_max", "ev_extend",
+            "Te", "dt", "pvr", "pvt", "px", "atol", "rtol", "gmres_rsrt", "gmres_rsrt_iter")
+    O = fv.Fvm1d3v(fv.Params(**{k: tp[k] for k in keys}), procs)
+    O.init(dt)
+    x = v.cpu().numpy()
+    O.rhs_mvec(E, x, dt)
+    t0 = time.perf_counter()
+    n = 2
+    for _ in range(n):
+        O.step_op_split(E, O.rhs_mvec(E, x, dt).reshape(x.shape), dt)
+    el = (time.perf_counter() - t0) / n
+    return {"value": 1.0 / (el * iters), "unit": "steps/s", "cores": len(_os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"{n} GMRES iterations (step operator + XVTVRC preconditioner) of the numpy oracle at the "
+                      f"full size, scaled by {iters:.1f} iterations per step"}
+
+
 def run_serving(N, rate):
     """Mixed 0D requests served in real time (boltzmann_amd/serve0d.py): Poisson arrivals at
     `rate`, each request drawing Nr in {64, 128, 256}, lmax in {1, 2, 4}, 1-3 processes,
@@ -514,7 +598,7 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="steady", choices=["steady", "ee", "transient", "glow", "serve"],
+    ap.add_argument("--workload", default="steady", choices=["steady", "ee", "transient", "glow", "serve", "fvm"],
                     help="steady: the batched 0D steady state (default); ee, transient, glow, serve: the other paths")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: mean arrival rate (requests/s)")
@@ -544,7 +628,7 @@ def main():
               f"trivially across GPUs; this build measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "steady":
-        {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow, "serve": bench_serve}[args.workload](args)
+        {"ee": bench_ee, "transient": bench_transient, "glow": bench_glow, "serve": bench_serve, "fvm": bench_fvm}[args.workload](args)
         return
 
     acc_idx = np.sort(np.random.default_rng(SEED + 1).choice(B, ACC_SAMPLE, replace=False))

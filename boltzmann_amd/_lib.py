@@ -54,6 +54,15 @@ class GlowFluid(ctypes.Structure):
                 ("DppTile", c_void_p)]
 
 
+
+class Fvm1d3v(ctypes.Structure):
+    _fields_ = ([("Nr", c_int), ("Nvt", c_int), ("Nx", c_int), ("nvr", c_int), ("nvt", c_int), ("nx", c_int)] +
+                [(k, c_void_p) for k in ("cos_vt", "sin_vt", "vr", "wq", "loss", "g_ptr", "g_col", "g_val", "dvrL",
+                                         "dvrR", "dvtL", "dvtR", "dxL", "dxR", "adx")] +
+                [(k, c_double) for k in ("dt_x", "tau_n0np0", "tau", "eq", "eaE")] +
+                [(k, c_void_p) for k in ("vr_l2r", "vr_r2l", "vt_l2r", "vt_r2l", "x_l2r", "x_r2l", "axvr", "ainv",
+                                         "H", "h_lo", "work")])
+
 # name -> argtypes codes ('p' pointer, 'i' int, 'd' double, 's' stream), read from include/bz.h;
 # every entry point returns int
 _SIGS = {}

@@ -13,7 +13,7 @@ INCLUDE = os.path.abspath(os.path.join(HERE, "..", "include"))
 
 SOURCES = ["bz_assembly.hip", "bz_solver0d.hip", "bz_solver0d_w8.hip", "bz_solver0d_w11.hip", "bz_solver0d_w12.hip",
            "bz_solver0d_w17.hip", "bz_solver0d_w20.hip", "bz_banded.hip", "bz_krylov.hip", "bz_ee.hip", "bz_noalloc.hip",
-           "bz_plan0d.hip", "bz_vstep.hip", "bz_glow.hip"]
+           "bz_plan0d.hip", "bz_vstep.hip", "bz_glow.hip", "bz_fvm.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

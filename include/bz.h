@@ -503,6 +503,66 @@ int bz_glow_fluid_step(const BzGlowFluid* g, int flags, double h, double bc_befo
 int bz_glow_tile_tables(int n, const double* DpT, const double* DppT, double* DpTile, double* DppTile,
                         hipStream_t stream);
 
+/* ------------------------------------------------------------------ fully implicit 1D3V (FVM) */
+
+/* Operators of the fully implicit 1D3V step on the finite-volume / ordinate velocity grid
+ * (bte_1d3v_solver.py:401-505, 780-862, 1333-1502).  State v (Nr*Nvt, Nx): radial point r,
+ * ordinate t (theta descending, cos < 0 on the first half), point x, x fastest.  All
+ * pointers are device memory; tables are built once by the host. */
+typedef struct {
+  int Nr, Nvt, Nx;            /* radial points, ordinates (even), x points                  */
+  int nvr, nvt, nx;           /* stencil widths pvr+1, pvt+1, px+1 (each <= 8)              */
+  const double* cos_vt;       /* (Nvt)                                                      */
+  const double* sin_vt;       /* (Nvt)                                                      */
+  const double* vr;           /* (Nr)                                                       */
+  const double* wq;           /* (Nvt) 2 pi w_t: the isotropic moment s = sum_t wq_t v_t   */
+  const double* loss;         /* (Nr) loss coefficient of the collision operator, 4 pi sum  */
+                              /* of the processes' dcs                                      */
+  const int* g_ptr;           /* (Nr+1) CSR of the gain matrix G (summed over processes):   */
+  const int* g_col;           /* (C f)[r, t] = (G s)[r] - loss[r] f[r, t]                   */
+  const double* g_val;
+  const double* dvrL;         /* (nvr, Nr) [d, r] = Dvr_LtoR[r, r-d]  (rows, for the        */
+  const double* dvrR;         /* (nvr, Nr) [d, r] = Dvr_RtoL[r, r+d]   product)             */
+  const double* dvtL;         /* (nvt, Nvt) [d, t] = Dvt_LtoR[t, t-d]                       */
+  const double* dvtR;         /* (nvt, Nvt) [d, t] = Dvt_RtoL[t, t+d]                       */
+  const double* dxL;          /* (nx, Nx) [k, i] = DpL[i, i-k]: the x shift, theta <= pi/2  */
+  const double* dxR;          /* (nx, Nx) [k, i] = DpR[i, i+k]: theta > pi/2               */
+  const double* adx;          /* (Nr) vr vth tau / L                                        */
+  double dt_x;                /* time step of the x shift (gmres_pc_ax dt at init)           */
+  double tau_n0np0;           /* tau n0 np0                                                  */
+  double tau;                 /* tau                                                         */
+  double eq;                  /* (1 / vth) q / m_e: scale of the advection operator          */
+  double eaE;                 /* tau q / m_e / vth: E to the sweeps' field                   */
+  /* preconditioner (XVTVRC) */
+  const double* vr_l2r;       /* (nvr, Nr) sweep bands, [k, j] = D[j+k, j]                  */
+  const double* vr_r2l;       /* (nvr, Nr) [k, j] = D[j-k, j]                               */
+  const double* vt_l2r;       /* (nvt, Nvt) diag(sin) Dvt, same layout                      */
+  const double* vt_r2l;
+  const double* x_l2r;        /* (nx, Nx)                                                    */
+  const double* x_r2l;
+  const double* axvr;         /* (Nr) vth tau / L vr: speed of the x sweep                   */
+  const double* ainv;         /* (Nr) 1 / (1 + c loss), c = gmres_pc_cv dt tau n0 np0      */
+  const double* H;            /* (Nr, Nr) c diag(ainv) G (diag(1 + c loss) - c W G)^-1      */
+  const int* h_lo;            /* (Nr) first nonzero column of each row of H                  */
+  double* work;               /* scratch, 2 Nr Nx doubles                                     */
+} BzFvm1d3v;
+
+/* y = the fully implicit step operator applied to x: the x shift (I + dt_x adx cos D) x
+ * minus dt tau (n0 np0 C x + E(x) eq A_E x), A_E the upwinded radial and angular advection
+ * chosen by the sign of E(x) (Ep for E >= 0, En otherwise), the v-space term zeroed at the
+ * inflow points (x = -1 for theta <= pi/2, x = 1 otherwise).  E: (Nx) in V/m.  Replaces
+ * rhs_mvec of bte_1d3v.step (bte_1d3v_solver.py:1902-1931). */
+int bz_fvm_rhs_mvec(const BzFvm1d3v* op, const double* E, double dt, const double* x, double* y,
+                    hipStream_t stream);
+
+/* y = the XVTVRC preconditioner applied to b: the collision solve (I - c C)^-1 (structured:
+ * isotropic moment, H, diagonal), then backward-Euler upwind sweeps in vr, vt and x with
+ * step alpha_dt, the inflow values zeroed before the x sweep.  y may not alias b.  Replaces
+ * step_op_split(..., XVTVRC) (bte_1d3v_solver.py:1745-1870), whose collision stage applies
+ * the dense PmatC or SpSM. */
+int bz_fvm_precond_xvtvrc(const BzFvm1d3v* op, const double* E, double alpha_dt, const double* b, double* y,
+                          hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
