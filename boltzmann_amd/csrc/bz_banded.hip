@@ -43,7 +43,6 @@ struct SweepArgs {
   int bsz, N, lower;
   int Nr, Nvt, Nx;
   const int* lower_of;   // K_XSTEP: direction of system s is lower_of[s % Nvt]
-  int zero_inflow;       // K_ADV_X: take the first value of every sweep as zero
 };
 
 // advection speed and direction of system s (rawkernel.py:183-192, 249-254, 316-322)
@@ -128,10 +127,8 @@ __global__ __launch_bounds__(TS) void k_sweep(SweepArgs a) {
       for (int q = 0; q < len; ++q, ++p) {
         const int o = fwd ? q : (len - 1 - q);                 // tile column
         const int c = fwd ? (k * CH + o) : (N - k * CH - len + o);   // matrix row
-        // the x-step (and the x sweep of the fully implicit preconditioner) zeroes the
-        // inflow value of every system before its sweep
-        const bool inflow = (KIND == K_XSTEP || (KIND == K_ADV_X && a.zero_inflow)) && p == 0;
-        const double bi = inflow ? 0.0 : tb[lane * ST + o];
+        // the x-step zeroes the inflow value of every system before its sweep
+        const double bi = (KIND == K_XSTEP && p == 0) ? 0.0 : tb[lane * ST + o];
         double val;
         if (BIDIAG) {
           const double sv = (p >= 1) ? tc[(TS + lane) * ST + o] * y[0] : 0.0;
@@ -417,16 +414,3 @@ int bz_bsr_upper_unit_solve(const double* data, const int* indptr, const int* in
 }
 
 }  // extern "C"
-
-// The x sweep of the fully implicit 1D3V preconditioner: adv_x_BE on (Nr*Nvt, Nx) with the
-// inflow value of every system zeroed first (bte_1d3v_solver.py:1849-1855).  Internal to
-// the library (bz_fvm.hip).
-int bz_internal_adv_x_be_inflow0(const double* DL, const double* DR, const double* axvr, const double* cos_vt,
-                                 double dt, const double* b, double* out, int ndiags, int Nr, int Nvt, int Nx,
-                                 hipStream_t stream) {
-  if (Nr < 1 || Nvt < 1 || Nx < 1 || ndiags < 1 || ndiags > 8) return BZ_EINVAL;
-  SweepArgs a{};
-  a.DL = DL; a.DR = DR; a.p0 = axvr; a.p1 = cos_vt; a.dt = dt; a.b = b; a.out = out;
-  a.Nr = Nr; a.Nvt = Nvt; a.Nx = Nx; a.bsz = Nr * Nvt; a.N = Nx; a.zero_inflow = 1;
-  return launch_sweep<K_ADV_X>(a, ndiags, stream);
-}

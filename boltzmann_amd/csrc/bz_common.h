@@ -151,9 +151,3 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 }  // namespace bz
-
-// x sweep with zeroed inflow values, shared by the fully implicit 1D3V preconditioner
-// (defined in bz_banded.hip).
-int bz_internal_adv_x_be_inflow0(const double* DL, const double* DR, const double* axvr, const double* cos_vt,
-                                 double dt, const double* b, double* out, int ndiags, int Nr, int Nvt, int Nx,
-                                 hipStream_t stream);

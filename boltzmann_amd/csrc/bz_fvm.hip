@@ -88,92 +88,307 @@ __global__ __launch_bounds__(256) void k_fvm_matvec(BzFvm1d3v op, const double* 
   y[e] = out;
 }
 
-// h[r, x] = sum_{r' >= h_lo[r]} H[r, r'] s[r', x]; RB rows per thread
-constexpr int RB = 4;
+// h[r, x] = sum_{r' >= h_lo[r]} H[r, r'] s[r', x]: one output per thread (consecutive lanes
+// take consecutive x, so H[r, r'] is one address for most of a wavefront and s rows are read
+// coalesced), the r' loop unrolled eight deep with independent partial sums so that eight
+// loads of each operand are in flight.
 __global__ __launch_bounds__(256) void k_fvm_coll_corr(const double* __restrict__ H, const int* __restrict__ h_lo,
                                                        const double* __restrict__ s, int Nr, int Nx,
                                                        double* __restrict__ h) {
-  const int nrb = (Nr + RB - 1) / RB;
   const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= nrb * Nx) return;
-  const int rb = e / Nx, ix = e - rb * Nx;
-  const int r0 = rb * RB;
-  int lo = Nr;
-  for (int j = 0; j < RB; ++j)
-    if (r0 + j < Nr) lo = min(lo, h_lo[r0 + j]);
-  double acc[RB];
+  if (e >= Nr * Nx) return;
+  const int r = e / Nx, ix = e - r * Nx;
+  const double* Hr = H + (size_t)r * Nr;
+  const double* sc = s + ix;
+  double acc[8];
 #pragma unroll
-  for (int j = 0; j < RB; ++j) acc[j] = 0.0;
-  for (int c = lo; c < Nr; ++c) {
-    const double sv = s[(size_t)c * Nx + ix];
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+  int c = h_lo[r];
+  for (; c + 8 <= Nr; c += 8) {
 #pragma unroll
-    for (int j = 0; j < RB; ++j)
-      if (r0 + j < Nr) acc[j] += H[(size_t)(r0 + j) * Nr + c] * sv;
+    for (int j = 0; j < 8; ++j) acc[j] += Hr[c + j] * sc[(size_t)(c + j) * Nx];
   }
-#pragma unroll
-  for (int j = 0; j < RB; ++j)
-    if (r0 + j < Nr) h[(size_t)(r0 + j) * Nx + ix] = acc[j];
+  for (; c < Nr; ++c) acc[0] += Hr[c] * sc[(size_t)c * Nx];
+  h[e] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
-// Backward-Euler upwind sweep along r (VR) or t (!VR) of one system per thread, in the state
-// layout: VR systems (t, x), elements Nvt*Nx apart; angular systems (r, x), elements Nx apart.
-// Same recurrence and operation order as rawkernel.adv_vr_BE_solve / adv_vt_BE_solve
-// (rawkernel.py:169-298): out_c = (b_c - av dt sum_d D[d, c-/+d] out_(c-/+d)) /
-// (1 + av dt D[0, c]), direction by the sign of the speed.  The radial sweep reads the
-// collision solve's output b / A + h on the fly.
-template <bool VR, int ND>
-__global__ __launch_bounds__(256) void k_fvm_sweep(BzFvm1d3v op, const double* __restrict__ E, double dt,
+// Backward-Euler upwind sweeps of the preconditioner, one system per thread, in the state
+// layout: radial systems (t, x) with elements Nvt*Nx apart, angular systems (r, x) with
+// elements Nx apart, x systems (r, t) contiguous.  The recurrence of
+// rawkernel.adv_vr_BE_solve / adv_vt_BE_solve / adv_x_BE_solve (rawkernel.py:169-375):
+// out_c = (b_c - av dt sum_d D[d, c-/+d] out_(c-/+d)) / (1 + av dt D[0, c]), direction by the
+// sign of the speed; the division is a product with the reciprocal, formed ahead of the
+// recurrence (the results differ from the division's in the last bit).  The radial sweep reads the collision solve's output b / A + h on the
+// fly; the x sweep takes the inflow value of every system as zero.  The operands of the next
+// PF positions are loaded ahead of the recurrence (they do not depend on it), and the
+// shared band tables sit in LDS when they fit, so each step waits on arithmetic only.
+enum { SW_VR = 0, SW_VT = 1, SW_X = 2 };
+constexpr int SWT = 64;
+constexpr int SW_LDS_DOUBLES = 8192;
+
+template <int MODE, int ND, bool LDS>
+__global__ __launch_bounds__(SWT) void k_fvm_sweep(BzFvm1d3v op, const double* __restrict__ E, double dt,
                                                    const double* b, double* out,   // may alias
                                                    const double* __restrict__ h) {
+  extern __shared__ double sm[];
+  constexpr int PF = (MODE == SW_VR && ND <= 4) ? 16 : 8;   // the radial sweep is the longest
   const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
-  const int s = blockIdx.x * 256 + threadIdx.x;
-  const int nsys = VR ? Nvt * Nx : Nr * Nx;
+  const int N = MODE == SW_VR ? Nr : (MODE == SW_VT ? Nvt : Nx);
+  const double* DLt = MODE == SW_VR ? op.vr_l2r : (MODE == SW_VT ? op.vt_l2r : op.x_l2r);
+  const double* DRt = MODE == SW_VR ? op.vr_r2l : (MODE == SW_VT ? op.vt_r2l : op.x_r2l);
+  const double* ainv = op.ainv;
+  if (LDS) {
+    for (int i = threadIdx.x; i < ND * N; i += SWT) {
+      sm[i] = DLt[i];
+      sm[ND * N + i] = DRt[i];
+    }
+    if (MODE == SW_VR)
+      for (int i = threadIdx.x; i < N; i += SWT) sm[2 * ND * N + i] = op.ainv[i];
+    __syncthreads();
+    DLt = sm;
+    DRt = sm + ND * N;
+    ainv = sm + 2 * ND * N;
+  }
+  const int s = blockIdx.x * SWT + threadIdx.x;
+  const int nsys = MODE == SW_VR ? Nvt * Nx : (MODE == SW_VT ? Nr * Nx : Nr * Nvt);
   if (s >= nsys) return;
-  const int ix = s % Nx, o = s / Nx;            // o = t (VR) or r
-  const double aE = op.eaE * E[ix];
   double av;
   bool fwd;
-  if (VR) { av = -aE * op.cos_vt[o]; fwd = av >= 0; }
-  else { av = aE / op.vr[o]; fwd = -av >= 0; }
-  const int N = VR ? Nr : Nvt;
-  const long base = VR ? (long)o * Nx + ix : (long)o * Nvt * Nx + ix;
-  const long stride = VR ? (long)Nvt * Nx : (long)Nx;
-  const double* D = VR ? (fwd ? op.vr_l2r : op.vr_r2l) : (fwd ? op.vt_l2r : op.vt_r2l);
+  long base, stride;
+  int ix = 0;
+  if (MODE == SW_VR) {
+    ix = s % Nx;
+    const int t = s / Nx;
+    const double aE = op.eaE * E[ix];
+    av = -aE * op.cos_vt[t];
+    fwd = av >= 0;
+    base = (long)t * Nx + ix;
+    stride = (long)Nvt * Nx;
+  } else if (MODE == SW_VT) {
+    ix = s % Nx;
+    const int r = s / Nx;
+    const double aE = op.eaE * E[ix];
+    av = aE / op.vr[r];
+    fwd = -av >= 0;
+    base = (long)r * Nvt * Nx + ix;
+    stride = Nx;
+  } else {
+    const int r = s / Nvt, t = s % Nvt;
+    av = op.axvr[r] * op.cos_vt[t];
+    fwd = av >= 0;
+    base = (long)s * Nx;
+    stride = 1;
+  }
+  const double* D = fwd ? DLt : DRt;
+  const double dtav = dt * av;
+  // operands of the next PF positions: b (and h), and the band entries of the position
+  double pb[PF], ph[PF], pd[PF][ND];
+  auto fetch = [&](int i, int pn) __attribute__((always_inline)) {
+    const int cn = fwd ? pn : N - 1 - pn;
+    pb[i] = b[base + cn * stride];
+    if (MODE == SW_VR) ph[i] = h[(size_t)cn * Nx + ix];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) pd[i][d] = D[d * N + (fwd ? max(cn - d, 0) : min(cn + d, N - 1))];
+  };
+#pragma unroll
+  for (int i = 0; i < PF; ++i) fetch(i, min(i, N - 1));
   double y[ND > 1 ? ND - 1 : 1];
 #pragma unroll
   for (int d = 0; d < (ND > 1 ? ND - 1 : 1); ++d) y[d] = 0.0;
-  for (int p = 0; p < N; ++p) {
+  // One position of the sweep; the prefetch index is clamped (the last position is loaded
+  // again near the end) and neighbours before the start of the sweep multiply y = 0 through
+  // a clamped table index, so the body has no branches.
+  auto step = [&](int i, int p) __attribute__((always_inline)) {
     const int c = fwd ? p : N - 1 - p;
-    const long idx = base + c * stride;
-    const double bi = VR ? b[idx] * op.ainv[c] + h[(size_t)c * Nx + ix] : b[idx];
+    double bi = pb[i];
+    if (MODE == SW_VR) bi = bi * ainv[c] + ph[i];
+    if (MODE == SW_X) bi = (p == 0) ? 0.0 : bi;
+    double dd[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) dd[d] = pd[i][d];
+    fetch(i, min(p + PF, N - 1));
+    // older neighbours first, so that only the last product waits on the previous value
     double sv = 0.0;
 #pragma unroll
-    for (int d = 1; d < ND; ++d)
-      if (d <= p) sv += D[d * N + (fwd ? c - d : c + d)] * y[d - 1];
-    const double term = !fwd ? sv * dt * av : (VR ? av * dt * sv : sv * av * dt);
-    const double val = (bi - term) / (1.0 + av * dt * D[c]);
+    for (int d = ND - 1; d >= 1; --d) sv += dd[d] * y[d - 1];
+    const double rden = 1.0 / (1.0 + av * dt * dd[0]);   // off the recurrence
+    const double val = (bi - sv * dtav) * rden;
 #pragma unroll
     for (int d = (ND > 1 ? ND - 2 : 0); d >= 1; --d) y[d] = y[d - 1];
     y[0] = val;
-    out[idx] = val;
+    out[base + c * stride] = val;
+  };
+  const int nfull = N - N % PF;
+  for (int p0 = 0; p0 < nfull; p0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) step(i, p0 + i);
+  }
+#pragma unroll
+  for (int i = 0; i < PF; ++i)
+    if (nfull + i < N) step(i, nfull + i);
+}
+
+// The radial sweep as a segmented scan.  A radial system has Nr positions (hundreds) but
+// there are only Nvt*Nx systems, so one thread per system leaves most of the chip idle and
+// waits on a long chain.  Here each system is cut into NSEG segments, one thread each.  Along
+// the sweep the state (the last K = ND-1 values) obeys an affine map per position,
+// y_new = alpha - sum_d beta_d y_(d-1) (alpha = b rden, beta_d = av dt D_d rden); pass 1 composes
+// the maps of its segment (K x K matrix A and offset c), the entering state of a segment is
+// the composition of the earlier segments applied to the zero state the sweep starts from
+// (through LDS), and pass 2 re-runs the segment with the sequential formula from that state.
+// The maps are contractions (upwind, 1 + av dt D_0 >= 1 in the direction of the sweep), so
+// the re-association changes the result at rounding level.  A workgroup holds SB systems
+// (consecutive x) times NSEG segments, so loads of one segment are coalesced over x.
+constexpr int SB = 16, NSEG = 16, SCAN_T = SB * NSEG;
+
+template <int ND>
+__global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const double* __restrict__ E, double dt,
+                                                        const double* __restrict__ b, double* __restrict__ out,
+                                                        const double* __restrict__ h) {
+  constexpr int K = ND > 1 ? ND - 1 : 1;
+  __shared__ double maps[SCAN_T][K * K + K];
+  const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
+  const int N = Nr;
+  const int j = threadIdx.x % SB, seg = threadIdx.x / SB;
+  const int s = blockIdx.x * SB + j;
+  const bool active = s < Nvt * Nx;
+  const int ss = active ? s : 0;
+  const int ix = ss % Nx, t = ss / Nx;
+  const double aE = op.eaE * E[ix];
+  const double av = -aE * op.cos_vt[t];
+  const bool fwd = av >= 0;
+  const double dtav = dt * av;
+  const long base = (long)t * Nx + ix, stride = (long)Nvt * Nx;
+  const double* D = fwd ? op.vr_l2r : op.vr_r2l;
+  const int L = (N + NSEG - 1) / NSEG;
+  const int p_lo = min(seg * L, N), p_hi = min(p_lo + L, N);
+  auto coeffs = [&](int p, double& bi, double dd[ND]) __attribute__((always_inline)) {
+    const int c = fwd ? p : N - 1 - p;
+    bi = b[base + c * stride] * op.ainv[c] + h[(size_t)c * Nx + ix];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) dd[d] = D[d * N + (fwd ? max(c - d, 0) : min(c + d, N - 1))];
+  };
+  // pass 1: the segment's map, state (y0 .. y_(K-1)), y0 the newest value
+  double A[K][K], cv[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    cv[r] = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[r][q] = (r == q) ? 1.0 : 0.0;
+  }
+  if (active) {
+    for (int p = p_lo; p < p_hi; ++p) {
+      double bi, dd[ND];
+      coeffs(p, bi, dd);
+      const double rden = 1.0 / (1.0 + av * dt * dd[0]);
+      double a0[K], c0 = bi * rden;
+#pragma unroll
+      for (int q = 0; q < K; ++q) a0[q] = 0.0;
+#pragma unroll
+      for (int d = ND - 1; d >= 1; --d) {
+        const double beta = dtav * dd[d] * rden;
+        c0 -= beta * cv[d - 1];
+#pragma unroll
+        for (int q = 0; q < K; ++q) a0[q] -= beta * A[d - 1][q];
+      }
+#pragma unroll
+      for (int r = K - 1; r >= 1; --r) {
+        cv[r] = cv[r - 1];
+#pragma unroll
+        for (int q = 0; q < K; ++q) A[r][q] = A[r - 1][q];
+      }
+      cv[0] = c0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) A[0][q] = a0[q];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    maps[threadIdx.x][K * K + r] = cv[r];
+#pragma unroll
+    for (int q = 0; q < K; ++q) maps[threadIdx.x][r * K + q] = A[r][q];
+  }
+  __syncthreads();
+  if (!active || p_lo >= p_hi) return;
+  // entering state: the earlier segments' maps applied in order to the zero state
+  double y[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) y[r] = 0.0;
+  for (int sg = 0; sg < seg; ++sg) {
+    const double* m = maps[sg * SB + j];
+    double yn[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      double acc = m[K * K + r];
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc += m[r * K + q] * y[q];
+      yn[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) y[r] = yn[r];
+  }
+  // pass 2: the sequential recurrence over the segment
+  for (int p = p_lo; p < p_hi; ++p) {
+    double bi, dd[ND];
+    coeffs(p, bi, dd);
+    double sv = 0.0;
+#pragma unroll
+    for (int d = ND - 1; d >= 1; --d) sv += dd[d] * y[d - 1];
+    const double rden = 1.0 / (1.0 + av * dt * dd[0]);
+    const double val = (bi - sv * dtav) * rden;
+#pragma unroll
+    for (int d = K - 1; d >= 1; --d) y[d] = y[d - 1];
+    y[0] = val;
+    out[base + (fwd ? p : N - 1 - p) * stride] = val;
   }
 }
 
-template <bool VR>
+template <int ND>
+void launch_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out, const double* h,
+                    hipStream_t st) {
+  const int nsys = op.Nvt * op.Nx;
+  hipLaunchKernelGGL((k_fvm_vr_scan<ND>), dim3((nsys + SB - 1) / SB), dim3(SCAN_T), 0, st, op, E, dt, b, out, h);
+}
+
+int launch_fvm_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out,
+                       const double* h, hipStream_t st) {
+  switch (op.nvr) {
+    case 2: launch_vr_scan<2>(op, E, dt, b, out, h, st); break;
+    case 3: launch_vr_scan<3>(op, E, dt, b, out, h, st); break;
+    case 4: launch_vr_scan<4>(op, E, dt, b, out, h, st); break;
+    case 5: launch_vr_scan<5>(op, E, dt, b, out, h, st); break;
+    default: return BZ_EINVAL;
+  }
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+template <int MODE, int ND>
+void launch_sweep_nd(const BzFvm1d3v& op, int N, int nsys, const double* E, double dt, const double* b, double* out,
+                     const double* h, hipStream_t st) {
+  const size_t lds = (size_t)(2 * ND + (MODE == SW_VR ? 1 : 0)) * N;
+  const dim3 grid((nsys + SWT - 1) / SWT), block(SWT);
+  if (lds <= SW_LDS_DOUBLES)
+    hipLaunchKernelGGL((k_fvm_sweep<MODE, ND, true>), grid, block, lds * sizeof(double), st, op, E, dt, b, out, h);
+  else
+    hipLaunchKernelGGL((k_fvm_sweep<MODE, ND, false>), grid, block, 0, st, op, E, dt, b, out, h);
+}
+
+template <int MODE>
 int launch_fvm_sweep(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out, const double* h,
                      hipStream_t st) {
-  const int nsys = VR ? op.Nvt * op.Nx : op.Nr * op.Nx;
-  const int nd = VR ? op.nvr : op.nvt;
-  const dim3 grid((nsys + 255) / 256), block(256);
+  const int N = MODE == SW_VR ? op.Nr : (MODE == SW_VT ? op.Nvt : op.Nx);
+  const int nsys = MODE == SW_VR ? op.Nvt * op.Nx : (MODE == SW_VT ? op.Nr * op.Nx : op.Nr * op.Nvt);
+  const int nd = MODE == SW_VR ? op.nvr : (MODE == SW_VT ? op.nvt : op.nx);
   switch (nd) {
-    case 1: hipLaunchKernelGGL((k_fvm_sweep<VR, 1>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 2: hipLaunchKernelGGL((k_fvm_sweep<VR, 2>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 3: hipLaunchKernelGGL((k_fvm_sweep<VR, 3>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 4: hipLaunchKernelGGL((k_fvm_sweep<VR, 4>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 5: hipLaunchKernelGGL((k_fvm_sweep<VR, 5>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 6: hipLaunchKernelGGL((k_fvm_sweep<VR, 6>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 7: hipLaunchKernelGGL((k_fvm_sweep<VR, 7>), grid, block, 0, st, op, E, dt, b, out, h); break;
-    case 8: hipLaunchKernelGGL((k_fvm_sweep<VR, 8>), grid, block, 0, st, op, E, dt, b, out, h); break;
+    case 1: launch_sweep_nd<MODE, 1>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 2: launch_sweep_nd<MODE, 2>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 3: launch_sweep_nd<MODE, 3>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 4: launch_sweep_nd<MODE, 4>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 5: launch_sweep_nd<MODE, 5>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 6: launch_sweep_nd<MODE, 6>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 7: launch_sweep_nd<MODE, 7>(op, N, nsys, E, dt, b, out, h, st); break;
+    case 8: launch_sweep_nd<MODE, 8>(op, N, nsys, E, dt, b, out, h, st); break;
     default: return BZ_EINVAL;
   }
   BZ_LAUNCH_CHECK();
@@ -227,16 +442,19 @@ int bz_fvm_precond_xvtvrc(const BzFvm1d3v* op, const double* E, double alpha_dt,
   hipLaunchKernelGGL(k_fvm_moment, dim3((nrx + 255) / 256), dim3(256), 0, stream, b, op->wq, op->Nr, op->Nvt, op->Nx,
                      s);
   BZ_LAUNCH_CHECK();
-  const int nrb = (op->Nr + RB - 1) / RB;
-  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((nrb * op->Nx + 255) / 256), dim3(256), 0, stream, op->H, op->h_lo, s,
-                     op->Nr, op->Nx, h);
+  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((nrx + 255) / 256), dim3(256), 0, stream, op->H, op->h_lo, s, op->Nr,
+                     op->Nx, h);
   BZ_LAUNCH_CHECK();
-  rc = launch_fvm_sweep<true>(*op, E, alpha_dt, b, y, h, stream);
+  // the radial sweep: segmented scan for the common stencils and long radial lines, else one
+  // thread per line
+  if (op->nvr >= 2 && op->nvr <= 5 && op->Nr >= 4 * NSEG)
+    rc = launch_fvm_vr_scan(*op, E, alpha_dt, b, y, h, stream);
+  else
+    rc = launch_fvm_sweep<SW_VR>(*op, E, alpha_dt, b, y, h, stream);
   if (rc != BZ_OK) return rc;
-  rc = launch_fvm_sweep<false>(*op, E, alpha_dt, y, y, nullptr, stream);
+  rc = launch_fvm_sweep<SW_VT>(*op, E, alpha_dt, y, y, nullptr, stream);
   if (rc != BZ_OK) return rc;
-  return bz_internal_adv_x_be_inflow0(op->x_l2r, op->x_r2l, op->axvr, op->cos_vt, alpha_dt, y, y, op->nx, op->Nr,
-                                      op->Nvt, op->Nx, stream);
+  return launch_fvm_sweep<SW_X>(*op, E, alpha_dt, y, y, nullptr, stream);
 }
 
 }  // extern "C"

@@ -46,7 +46,8 @@ def _field(x, t=0.25):
     return 1e4 * x ** 3 * np.sin(2 * np.pi * t)
 
 
-CASES = [dict(), dict(pvr=2, pvt=2, px=2, Nvt=12, ev_extend=2, Np=9)]
+# the third case has enough radial points (65) for the segmented-scan radial sweep
+CASES = [dict(), dict(pvr=2, pvt=2, px=2, Nvt=12, ev_extend=2, Np=9), dict(spline_qpts=5, Np=10)]
 
 
 @pytest.mark.parametrize("kw", CASES)
