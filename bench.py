@@ -507,6 +507,27 @@ def bench_fvm(args):
     torch.cuda.synchronize()
     dt_w = time.perf_counter() - t0
     q = S.compute_qoi(v)
+    # roofline of the step operator (bz_fvm_rhs_mvec: the isotropic moment and the stencil
+    # kernel), timed with HIP events on the stream its kernels are launched on; algorithmic
+    # bytes per call: the state read once and the product written once
+    E = Ef(t)
+    xf = v.reshape(-1).contiguous()
+    for _ in range(3):
+        S.rhs_mvec(E, xf, dt)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    nrep = 50
+    e0.record(st)
+    for _ in range(nrep):
+        S.rhs_mvec(E, xf, dt)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_op = e0.elapsed_time(e1) * 1e-3 / nrep
+    alg = 2.0 * 8 * S.dof_v * S.dof_x
+    roof = {"bound": "hbm", "achieved": alg / t_op / 1e9, "peak": 8000.0, "unit": "GB/s",
+            "frac": alg / t_op / 1e9 / 8000.0, "traffic": None,
+            "kernel": "bz_fvm_rhs_mvec (k_fvm_moment + k_fvm_matvec)", "launch_us": t_op * 1e6,
+            "algorithmic_bytes": alg}
     cpu = None
     if not args.no_cpu:
         cpu = fvm_cpu_baseline(tp, S, v, Ef(t).cpu().numpy(), dt, float(np.mean(iters)))
@@ -520,7 +541,7 @@ def bench_fvm(args):
                    "dof": S.dof_v * S.dof_x, "Nr": S.dof_vr, "Nvt": S.dof_vt, "Nx": S.dof_x},
         "gmres_iterations_mean": float(np.mean(iters)), "gmres_iterations_max": int(np.max(iters)),
         "ne_mean_m3": float(q["ne"].mean()), "Te_mean_eV": float(q["Te"].mean()),
-        "rf_periods_simulated": t - 0.25, "setup_s": setup_s, "cpu_baseline": cpu}), flush=True)
+        "rf_periods_simulated": t - 0.25, "setup_s": setup_s, "roofline": roof, "cpu_baseline": cpu}), flush=True)
 
 
 def fvm_cpu_baseline(tp, S, v, E, dt, iters):

@@ -38,11 +38,13 @@ __global__ __launch_bounds__(256) void k_fvm_matvec(BzFvm1d3v op, const double* 
                                                     const double* __restrict__ x, const double* __restrict__ s,
                                                     double* __restrict__ y) {
   const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)Nr * Nvt * Nx) return;
-  const int ix = (int)(e % Nx);
-  const int rt = (int)(e / Nx);
-  const int t = rt % Nvt, r = rt / Nvt;
+  // 32-bit index arithmetic (the size is checked below 2^31 by the host)
+  const unsigned eu = blockIdx.x * 256u + threadIdx.x;
+  if (eu >= (unsigned)(Nr * Nvt * Nx)) return;
+  const long e = eu;
+  const unsigned rt = eu / (unsigned)Nx;
+  const int ix = (int)(eu - rt * (unsigned)Nx);
+  const int r = (int)(rt / (unsigned)Nvt), t = (int)(rt - (unsigned)r * (unsigned)Nvt);
   const int mp = Nvt / 2;
   const double cs = op.cos_vt[t];
   const bool low = t >= mp;                    // theta <= pi/2: cos > 0, left-to-right in x
