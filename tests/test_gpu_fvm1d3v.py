@@ -91,9 +91,13 @@ def test_preconditioner_matches_oracle(kw):
         assert _rel(got, want) < 1e-12, _rel(got, want)
 
 
+@pytest.mark.parametrize("device_gmres", [True, False])
 @pytest.mark.parametrize("kw", CASES)
-def test_step_matches_dense_solve(kw):
+def test_step_matches_dense_solve(kw, device_gmres):
+    """Both GMRES drivers: the device one (graph-replayed Arnoldi steps, true residual per
+    cycle) and the reference's loop (true residual after every step)."""
     S, O = _pair(**kw)
+    S.use_device_gmres = device_gmres
     dt = 1e-3
     S.init(dt)
     O.init(dt)
@@ -118,7 +122,7 @@ def test_time_steps_and_moments():
     for k in range(8):
         E = _field(O.xp, 0.25 + k * dt)
         v = O.step(E, v, dt)
-        vd = S.step(S._field(E), vd, None, k * dt, dt)
+        vd = S.step(S._field(E), vd, None, k * dt, dt)     # replays the captured GMRES steps
         assert _rel(vd, v) < 1e-8, (k, _rel(vd, v))
     q, qd = O.compute_qoi(v), S.compute_qoi(vd)
     for key in ("ne", "Te"):
@@ -126,8 +130,10 @@ def test_time_steps_and_moments():
     assert _rel(qd["rates"], q["rates"]) < 1e-8
 
 
-def test_gmres_failure_raises():
+@pytest.mark.parametrize("device_gmres", [True, False])
+def test_gmres_failure_raises(device_gmres):
     S, O = _pair()
+    S.use_device_gmres = device_gmres
     dt = 1e-3
     S.params.gmres_maxiter = 1
     S.params.rtol = 1e-30
