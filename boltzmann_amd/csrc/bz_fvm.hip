@@ -90,28 +90,55 @@ __global__ __launch_bounds__(256) void k_fvm_matvec(BzFvm1d3v op, const double* 
   y[e] = out;
 }
 
-// h[r, x] = sum_{r' >= h_lo[r]} H[r, r'] s[r', x]: one output per thread (consecutive lanes
-// take consecutive x, so H[r, r'] is one address for most of a wavefront and s rows are read
-// coalesced), the r' loop unrolled eight deep with independent partial sums so that eight
-// loads of each operand are in flight.
+// h = H s on the matrix cores: one workgroup of four wavefronts per 16 x 16 tile of h (rows
+// r, points x).  The columns r' run from the first nonzero column of the tile's rows (H is
+// upper triangular: collisions only move electrons down in energy) to Nr in steps of 4, split
+// in four ranges, one per wavefront, each accumulated by v_mfma_f64_16x16x4f64; the four
+// partial tiles are summed through LDS.  Operand layout of the instruction (measured on
+// gfx950): A (16 x 4) lane = i + 16 k, B (4 x 16) lane = j + 16 k; element m of the
+// accumulator of lane l is D[4 m + l / 16][l % 16].  Rows and points beyond the ends are
+// clamped on load and not stored.
+typedef double bz_d4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_fvm_coll_corr(const double* __restrict__ H, const int* __restrict__ h_lo,
                                                        const double* __restrict__ s, int Nr, int Nx,
                                                        double* __restrict__ h) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= Nr * Nx) return;
-  const int r = e / Nx, ix = e - r * Nx;
-  const double* Hr = H + (size_t)r * Nr;
-  const double* sc = s + ix;
-  double acc[8];
+  __shared__ double red[3][256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+  const int row = min(r0 + i16, Nr - 1), col = min(c0 + i16, Nx - 1);
+  int kmin = h_lo[row];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
-  int c = h_lo[r];
-  for (; c + 8 <= Nr; c += 8) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) acc[j] += Hr[c + j] * sc[(size_t)(c + j) * Nx];
+  for (int o = 8; o >= 1; o >>= 1) kmin = min(kmin, __shfl_xor(kmin, o, 16));
+  kmin = __shfl(kmin, 0, 64);
+  const int k0 = kmin & ~3;
+  const int nsteps = (Nr - k0 + 3) >> 2;
+  const int per = (nsteps + 3) >> 2;
+  const int s_lo = wave * per, s_hi = min(nsteps, s_lo + per);
+  const double* Hr = H + (size_t)row * Nr;
+  bz_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int st = s_lo; st < s_hi; ++st) {
+    const int kk = k0 + 4 * st + k4;
+    const int kc = min(kk, Nr - 1);
+    const double a = kk < Nr ? Hr[kc] : 0.0;
+    const double bv = s[(size_t)kc * Nx + col];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
   }
-  for (; c < Nr; ++c) acc[0] += Hr[c] * sc[(size_t)c * Nx];
-  h[e] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  if (wave > 0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) red[wave - 1][m * 64 + lane] = acc[m];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double v = acc[m] + red[0][m * 64 + lane] + red[1][m * 64 + lane] + red[2][m * 64 + lane];
+      const int rr = r0 + 4 * m + k4, cc = c0 + i16;
+      if (rr < Nr && cc < Nx) h[(size_t)rr * Nx + cc] = v;
+    }
+  }
 }
 
 // Backward-Euler upwind sweeps of the preconditioner, one system per thread, in the state
@@ -241,7 +268,7 @@ __global__ __launch_bounds__(SWT) void k_fvm_sweep(BzFvm1d3v op, const double* _
 // The maps are contractions (upwind, 1 + av dt D_0 >= 1 in the direction of the sweep), so
 // the re-association changes the result at rounding level.  A workgroup holds SB systems
 // (consecutive x) times NSEG segments, so loads of one segment are coalesced over x.
-constexpr int SB = 16, NSEG = 16, SCAN_T = SB * NSEG;
+constexpr int SB = 8, NSEG = 32, SCAN_T = SB * NSEG;
 
 template <int ND>
 __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const double* __restrict__ E, double dt,
@@ -444,12 +471,12 @@ int bz_fvm_precond_xvtvrc(const BzFvm1d3v* op, const double* E, double alpha_dt,
   hipLaunchKernelGGL(k_fvm_moment, dim3((nrx + 255) / 256), dim3(256), 0, stream, b, op->wq, op->Nr, op->Nvt, op->Nx,
                      s);
   BZ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((nrx + 255) / 256), dim3(256), 0, stream, op->H, op->h_lo, s, op->Nr,
-                     op->Nx, h);
+  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((op->Nr + 15) / 16, (op->Nx + 15) / 16), dim3(256), 0, stream, op->H,
+                     op->h_lo, s, op->Nr, op->Nx, h);
   BZ_LAUNCH_CHECK();
   // the radial sweep: segmented scan for the common stencils and long radial lines, else one
   // thread per line
-  if (op->nvr >= 2 && op->nvr <= 5 && op->Nr >= 4 * NSEG)
+  if (op->nvr >= 2 && op->nvr <= 5 && op->Nr >= 2 * NSEG)
     rc = launch_fvm_vr_scan(*op, E, alpha_dt, b, y, h, stream);
   else
     rc = launch_fvm_sweep<SW_VR>(*op, E, alpha_dt, b, y, h, stream);
