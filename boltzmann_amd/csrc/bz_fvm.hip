@@ -90,6 +90,102 @@ __global__ __launch_bounds__(256) void k_fvm_matvec(BzFvm1d3v op, const double* 
   y[e] = out;
 }
 
+// The step operator for fixed stencil widths (the common ones): every neighbour load is issued
+// unconditionally from a clamped address and weighted by zero where the stencil has no entry,
+// so all loads of an element go out together instead of one loop iteration at a time; the
+// gain row is read the same way, GMAX entries at once.  Same sums, in the same order, as
+// k_fvm_matvec (the zero terms add exactly nothing).
+template <int NX, int NVR, int NVT, int GMAX>
+__global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double* __restrict__ E, double dt,
+                                                      const double* __restrict__ x, const double* __restrict__ s,
+                                                      double* __restrict__ y) {
+  const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
+  // Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8); the grid is a
+  // multiple of 8 and XCD k takes the k-th eighth of the state, so the radial neighbours
+  // (Nvt Nx elements away) of a workgroup's elements sit in its own XCD's L2.
+  const unsigned per = gridDim.x >> 3;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const unsigned eu = lb * 256u + threadIdx.x;
+  if (eu >= (unsigned)(Nr * Nvt * Nx)) return;
+  const long e = eu;
+  const unsigned rt = eu / (unsigned)Nx;
+  const int ix = (int)(eu - rt * (unsigned)Nx);
+  const int r = (int)(rt / (unsigned)Nvt), t = (int)(rt - (unsigned)r * (unsigned)Nvt);
+  const int mp = Nvt / 2;
+  const double cs = op.cos_vt[t];
+  const bool low = t >= mp;
+  const double cx = op.dt_x * op.adx[r] * cs;
+  const double* DX = low ? op.dxL : op.dxR;
+  double xv[NX], dx[NX];
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int j = min(max(low ? ix - k : ix + k, 0), Nx - 1);
+    xv[k] = x[e + (j - ix)];
+    dx[k] = DX[k * Nx + ix];
+  }
+  double yx = 0.0;
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int j = low ? ix - k : ix + k;
+    const double d = cx * dx[k] + (k == 0 ? 1.0 : 0.0);
+    yx += (j >= 0 && j < Nx) ? d * xv[k] : 0.0;
+  }
+  const bool inflow = low ? (ix == 0) : (ix == Nx - 1);
+  double out = yx;
+  if (!inflow) {
+    const int q0 = op.g_ptr[r], q1 = op.g_ptr[r + 1];
+    const int qmax = max(op.g_ptr[Nr] - 1, 0);
+    double gv[GMAX], gs[GMAX];
+#pragma unroll
+    for (int k = 0; k < GMAX; ++k) {
+      const int q = min(q0 + k, qmax);
+      gv[k] = op.g_val[q];
+      gs[k] = s[(size_t)op.g_col[q] * Nx + ix];
+    }
+    double g = 0.0;
+#pragma unroll
+    for (int k = 0; k < GMAX; ++k) g += (q0 + k < q1) ? gv[k] * gs[k] : 0.0;
+    for (int q = q0 + GMAX; q < q1; ++q) g += op.g_val[q] * s[(size_t)op.g_col[q] * Nx + ix];
+    const double xc = x[e];
+    const double cv = g - op.loss[r] * xc;
+    const double Ex = E[ix];
+    const bool ep = Ex >= 0;
+    const bool vr_up = (ep == low);
+    const double* Dr = vr_up ? op.dvrR : op.dvrL;
+    const long rs = (long)Nvt * Nx;
+    double xr[NVR], dr[NVR];
+#pragma unroll
+    for (int d = 0; d < NVR; ++d) {
+      const int j = min(max(vr_up ? r + d : r - d, 0), Nr - 1);
+      xr[d] = x[e + (long)(j - r) * rs];
+      dr[d] = Dr[d * Nr + r];
+    }
+    double ar = 0.0;
+#pragma unroll
+    for (int d = 0; d < NVR; ++d) {
+      const int j = vr_up ? r + d : r - d;
+      ar += (j >= 0 && j < Nr) ? dr[d] * xr[d] : 0.0;
+    }
+    const double* Dt = ep ? op.dvtR : op.dvtL;
+    double xt[NVT], dtt[NVT];
+#pragma unroll
+    for (int d = 0; d < NVT; ++d) {
+      const int j = min(max(ep ? t + d : t - d, 0), Nvt - 1);
+      xt[d] = x[e + (long)(j - t) * Nx];
+      dtt[d] = Dt[d * Nvt + t];
+    }
+    double at = 0.0;
+#pragma unroll
+    for (int d = 0; d < NVT; ++d) {
+      const int j = ep ? t + d : t - d;
+      at += (j >= 0 && j < Nvt) ? dtt[d] * xt[d] : 0.0;
+    }
+    const double ae = cs * ar - (op.sin_vt[t] / op.vr[r]) * at;
+    out = yx - dt * (op.tau_n0np0 * cv + op.tau * (Ex * (op.eq * ae)));
+  }
+  y[e] = out;
+}
+
 // h = H s on the matrix cores: one workgroup of four wavefronts per 16 x 16 tile of h (rows
 // r, points x).  The columns r' run from the first nonzero column of the tile's rows (H is
 // upper triangular: collisions only move electrons down in energy) to Nr in steps of 4, split
@@ -451,7 +547,14 @@ int bz_fvm_rhs_mvec(const BzFvm1d3v* op, const double* E, double dt, const doubl
                      s);
   BZ_LAUNCH_CHECK();
   const long n = (long)nrx * op->Nvt;
-  hipLaunchKernelGGL(k_fvm_matvec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *op, E, dt, x, s, y);
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  const dim3 grid8((unsigned)(((n + 255) / 256 + 7) / 8 * 8));     // XCD-blocked kernels
+  if (op->nx == 2 && op->nvr == 4 && op->nvt == 4)
+    hipLaunchKernelGGL((k_fvm_matvec_w<2, 4, 4, 8>), grid8, block, 0, stream, *op, E, dt, x, s, y);
+  else if (op->nx == 3 && op->nvr == 3 && op->nvt == 3)
+    hipLaunchKernelGGL((k_fvm_matvec_w<3, 3, 3, 8>), grid8, block, 0, stream, *op, E, dt, x, s, y);
+  else
+    hipLaunchKernelGGL(k_fvm_matvec, grid, block, 0, stream, *op, E, dt, x, s, y);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }
