@@ -46,12 +46,15 @@ QOI_TOL = 1e-8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_newton0d_b4096_n387_hbm_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per solver launch (k_prepare0d + k_newton0d) from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this workload, each scaled by the
+FVM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fvm1d3v_step_hbm_traffic.txt")
+
+
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of a
+    workload (default: the solver launch k_prepare0d + k_newton0d), each scaled by the
     counter's reading for a 1 GiB copy (the file states commands and factors)."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -525,7 +528,7 @@ def bench_fvm(args):
     t_op = e0.elapsed_time(e1) * 1e-3 / nrep
     alg = 2.0 * 8 * S.dof_v * S.dof_x
     roof = {"bound": "hbm", "achieved": alg / t_op / 1e9, "peak": 8000.0, "unit": "GB/s",
-            "frac": alg / t_op / 1e9 / 8000.0, "traffic": None,
+            "frac": alg / t_op / 1e9 / 8000.0, "traffic": measured_traffic(FVM_TRAFFIC_FILE),
             "kernel": "bz_fvm_rhs_mvec (k_fvm_moment + k_fvm_matvec)", "launch_us": t_op * 1e6,
             "algorithmic_bytes": alg}
     cpu = None

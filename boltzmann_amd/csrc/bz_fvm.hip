@@ -100,17 +100,18 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
                                                       const double* __restrict__ x, const double* __restrict__ s,
                                                       double* __restrict__ y) {
   const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
-  // Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8); the grid is a
-  // multiple of 8 and XCD k takes the k-th eighth of the state, so the radial neighbours
-  // (Nvt Nx elements away) of a workgroup's elements sit in its own XCD's L2.
+  // Workgroups of one state row (r, t) each, threads over x: the row indices are workgroup-
+  // uniform (no per-thread division).  Workgroups are dealt to the 8 XCDs in turn (workgroup
+  // b to XCD b % 8); the grid is a multiple of 8 and XCD k takes the k-th eighth of the rows,
+  // so the radial neighbours (Nvt rows away) of a row are read through the same L2.
+  const int bx = (Nx + blockDim.x - 1) / blockDim.x;            // workgroups per row
   const unsigned per = gridDim.x >> 3;
   const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const unsigned eu = lb * 256u + threadIdx.x;
-  if (eu >= (unsigned)(Nr * Nvt * Nx)) return;
-  const long e = eu;
-  const unsigned rt = eu / (unsigned)Nx;
-  const int ix = (int)(eu - rt * (unsigned)Nx);
-  const int r = (int)(rt / (unsigned)Nvt), t = (int)(rt - (unsigned)r * (unsigned)Nvt);
+  const int row = (int)(lb / (unsigned)bx);
+  const int ix = (int)(lb - (unsigned)row * bx) * blockDim.x + threadIdx.x;
+  if (row >= Nr * Nvt || ix >= Nx) return;
+  const int r = row / Nvt, t = row - r * Nvt;
+  const long e = (long)row * Nx + ix;
   const int mp = Nvt / 2;
   const double cs = op.cos_vt[t];
   const bool low = t >= mp;
@@ -548,11 +549,14 @@ int bz_fvm_rhs_mvec(const BzFvm1d3v* op, const double* E, double dt, const doubl
   BZ_LAUNCH_CHECK();
   const long n = (long)nrx * op->Nvt;
   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-  const dim3 grid8((unsigned)(((n + 255) / 256 + 7) / 8 * 8));     // XCD-blocked kernels
-  if (op->nx == 2 && op->nvr == 4 && op->nvt == 4)
-    hipLaunchKernelGGL((k_fvm_matvec_w<2, 4, 4, 8>), grid8, block, 0, stream, *op, E, dt, x, s, y);
-  else if (op->nx == 3 && op->nvr == 3 && op->nvt == 3)
-    hipLaunchKernelGGL((k_fvm_matvec_w<3, 3, 3, 8>), grid8, block, 0, stream, *op, E, dt, x, s, y);
+  const int tx = op->Nx <= 64 ? 64 : (op->Nx <= 128 ? 128 : 256);       // row kernels: x per workgroup
+  const long nwg = (long)op->Nr * op->Nvt * ((op->Nx + tx - 1) / tx);
+  const dim3 growx((unsigned)((nwg + 7) / 8 * 8)), browx(tx);
+  const bool rows_fit = nwg < (1L << 30);
+  if (rows_fit && op->nx == 2 && op->nvr == 4 && op->nvt == 4)
+    hipLaunchKernelGGL((k_fvm_matvec_w<2, 4, 4, 8>), growx, browx, 0, stream, *op, E, dt, x, s, y);
+  else if (rows_fit && op->nx == 3 && op->nvr == 3 && op->nvt == 3)
+    hipLaunchKernelGGL((k_fvm_matvec_w<3, 3, 3, 8>), growx, browx, 0, stream, *op, E, dt, x, s, y);
   else
     hipLaunchKernelGGL(k_fvm_matvec, grid, block, 0, stream, *op, E, dt, x, s, y);
   BZ_LAUNCH_CHECK();
