@@ -46,8 +46,11 @@ def _field(x, t=0.25):
     return 1e4 * x ** 3 * np.sin(2 * np.pi * t)
 
 
-# the third case has enough radial points (65) for the segmented-scan radial sweep
-CASES = [dict(), dict(pvr=2, pvt=2, px=2, Nvt=12, ev_extend=2, Np=9), dict(spline_qpts=5, Np=10)]
+# the third case has enough radial points (65) for the segmented-scan radial sweep; the last
+# two use stencil widths without a fixed-width step-operator kernel (the general one), the
+# 1.2x extended domain and two-point radial stencils in the scan
+CASES = [dict(), dict(pvr=2, pvt=2, px=2, Nvt=12, ev_extend=2, Np=9), dict(spline_qpts=5, Np=10),
+         dict(pvr=1, pvt=2, px=3, Nvt=10, Np=11, ev_extend=1), dict(pvr=1, spline_qpts=5, Np=8)]
 
 
 @pytest.mark.parametrize("kw", CASES)
