@@ -32,7 +32,7 @@ constexpr int GF_NPAD = 384;       // >= CG ceil(Np / CG) for every Np <= 200: p
 constexpr int BZ_GLOW_TILE_THREADS = 512;   // block size of the tile layouts
 constexpr int GF_TILE_RED = 5000;  // >= max(TR, TC) Npad of the tile layouts below (25 x 200 at 8 x 10)
 
-enum { V_NE = 0, V_NI, V_E, V_X, V_RR, V_R0, V_W, V_Z, V_DX, V_COUNT };
+enum { V_NE = 0, V_NI, V_E, V_X, V_RR, V_R0, V_W, V_Z, V_DX, V_DI, V_COUNT };
 
 struct alignas(16) GfShared {
   double vec[V_COUNT][GF_NPAD];
@@ -372,7 +372,10 @@ __device__ __forceinline__ bool gf_tile_build(const GfCtx& q, GfShared& S, const
       for (int i = 0; i < BR; ++i) {
         const int r = tl.tr * BR + i;
         if (r != c) sabs += fabs(X[i][j]);
-        else S.tprow[0][c] = fabs(X[i][j]);
+        else {
+          S.tprow[0][c] = fabs(X[i][j]);
+          S.vec[V_DI][c] = 1.0 / X[i][j];              // the Jacobi path's D^-1
+        }
       }
       S.red2[tl.tr * tl.Npad + c] = sabs;
     }
@@ -383,11 +386,43 @@ __device__ __forceinline__ bool gf_tile_build(const GfCtx& q, GfShared& S, const
     double sum = 0.0;
     for (int r = 0; r < tl.TR; ++r) sum += S.red2[r * tl.Npad + q.t];
     if (!(S.tprow[0][q.t] > sum)) S.flag = 0;
+    S.red[q.t] = sum / S.tprow[0][q.t];            // column ratio (padding columns: 0)
+  }
+  __syncthreads();
+  if (q.t == 0) {
+    double rmax = 0.0;
+    for (int c = 0; c < tl.Npad; ++c) rmax = fmax(rmax, S.red[c]);
+    S.sc[0] = rmax;
   }
   __syncthreads();
   const bool dom = S.flag != 0;
   __syncthreads();
   return dom;
+}
+
+template <int BR, int BC>
+__device__ __forceinline__ void gf_tile_apply(const GfCtx& q, GfShared& S, const GfTile& tl,
+                                              const double (&X)[BR][BC], const double* rr, double* dx);
+
+// dx = J^-1 rr by the Jacobi iteration dx <- dx + D^-1 (rr - J dx) from dx = D^-1 rr, on the
+// tile-resident J.  With rho the largest column ratio sum_{r != c} |J_rc| / |J_cc|, the
+// iteration matrix D^-1 (J - D) has norm rho in |D x|_1, and the start is already within rho
+// of the solution, so after K steps the error is below rho^(K+1) of |D dx|_1; K is chosen for
+// 2^-53.  Each step is one tile product (two barriers) and an update (one barrier), against
+// one barrier per elimination step for the Gauss-Jordan inverse.
+template <int BR, int BC>
+__device__ __forceinline__ void gf_tile_jacobi(const GfCtx& q, GfShared& S, const GfTile& tl,
+                                               const double (&X)[BR][BC], int K, const double* rr, double* dx) {
+  const int t = q.t;
+  const double* di = S.vec[V_DI];
+  double* w = S.vec[V_Z];
+  if (t < q.n) dx[t] = di[t] * rr[t];
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    gf_tile_apply<BR, BC>(q, S, tl, X, dx, w);
+    if (t < q.n) dx[t] = fma(di[t], rr[t] - w[t], dx[t]);
+    __syncthreads();
+  }
 }
 
 // In-place Gauss-Jordan inverse without row exchanges on the tiles.  All threads rotate their
@@ -532,9 +567,17 @@ __global__ __launch_bounds__(NT) void k_glow_fluid(BzGlowFluid g, int flags, dou
       const GfTile tl = gf_tile_ctx<BR, BC>(q);
       double Xt[BR][BC];
       if (gf_tile_build<BR, BC>(q, S, g, h, tl, Xt)) {
-        const int singular = gf_tile_gauss_jordan<BR, BC>(q, S, tl, Xt);
-        gf_newton(q, S, g, h, singular,
-                  [&](const double* rr, double* dx) { gf_tile_apply<BR, BC>(q, S, tl, Xt, rr, dx); });
+        const double rho = S.sc[0];
+        if (rho <= 0.5) {
+          // strongly dominant (the glow's half steps: rho about 0.14): Jacobi iterations on J
+          const int K = rho > 0.0 ? (int)ceil(36.7368005696771 / -log(rho)) + 1 : 1;   // 53 ln 2
+          gf_newton(q, S, g, h, 4,                                   // bit 2: Jacobi path
+                    [&](const double* rr, double* dx) { gf_tile_jacobi<BR, BC>(q, S, tl, Xt, K, rr, dx); });
+        } else {
+          const int singular = gf_tile_gauss_jordan<BR, BC>(q, S, tl, Xt);
+          gf_newton(q, S, g, h, singular,
+                    [&](const double* rr, double* dx) { gf_tile_apply<BR, BC>(q, S, tl, Xt, rr, dx); });
+        }
         done = true;
       }
     }

@@ -53,7 +53,11 @@ def test_fluid_launch_against_oracle(Np):
     s = 0.5 * (xp + 1)
     ref = gc.cheb_collocation_1d(Np)
     mu, Di = np.full(Np, p.mu_i), np.full(Np, p.Di)
-    for h, amp in ((5e-4, 1.0), (0.1, 300.0)):
+    # a third step length per tile-layout size makes J dominant with a column ratio between 0.5
+    # and 1 (host scan of the same J): the Gauss-Jordan tile path instead of the Jacobi one
+    h_mid = {100: 0.02, 137: 0.01, 200: 0.002}.get(Np)
+    cases = ((5e-4, 1.0), (0.1, 300.0)) + (((h_mid, 1.0),) if h_mid else ())
+    for h, amp in cases:
         print(f"case: fluid launch Np={Np} h={h} density x{amp}", flush=True)
         ne = amp * 1e6 * (1e7 + 1e9 * s ** 2 * (1 - s) ** 2) / p.np0 * (1 + 0.02 * np.cos(5 * xp))
         ni = ne * (1 + 0.03 * np.sin(3 * np.pi * xp))
@@ -65,11 +69,14 @@ def test_fluid_launch_against_oracle(Np):
         glow._fluid_launch(ud, GLOW_STEP | GLOW_FIELD_AFTER, h=h, t_before=t0, t_after=t1)
         torch.cuda.synchronize()
         st = glow._status.tolist()
-        print(f"  status {st} ({'pivoted' if st[3] & 2 else 'pivot-free'} inverse)", flush=True)
+        path = "pivoted inverse" if st[3] & 2 else ("Jacobi" if st[3] & 4 else "pivot-free inverse")
+        print(f"  status {st} ({path})", flush=True)
         assert st[0] == 1 and (st[3] & 1) == 0, st
         if h < 1e-3 and Np > 90:
-            assert (st[3] & 2) == 0, "the glow half step must take the pivot-free tile path"
-        if h > 1e-2 and Np == 200:
+            assert (st[3] & 6) == 4, "the glow half step must take the Jacobi tile path"
+        if h == h_mid:
+            assert (st[3] & 6) == 0, "a column ratio in (0.5, 1) must take the Gauss-Jordan tile path"
+        if h > 5e-2 and Np == 200:
             assert (st[3] & 2) != 0, "the long step must take the pivoted path"
 
         # the field is the derivative of a nearly linear potential: its rounding is bounded by
