@@ -408,19 +408,33 @@ __device__ __forceinline__ void gf_tile_apply(const GfCtx& q, GfShared& S, const
 // tile-resident J.  With rho the largest column ratio sum_{r != c} |J_rc| / |J_cc|, the
 // iteration matrix D^-1 (J - D) has norm rho in |D x|_1, and the start is already within rho
 // of the solution, so after K steps the error is below rho^(K+1) of |D dx|_1; K is chosen for
-// 2^-53.  Each step is one tile product (two barriers) and an update (one barrier), against
-// one barrier per elimination step for the Gauss-Jordan inverse.
+// 2^-53.  Each step is one tile product and an update, two barriers, against one barrier
+// per elimination step (and Npad steps) for the Gauss-Jordan inverse.
 template <int BR, int BC>
 __device__ __forceinline__ void gf_tile_jacobi(const GfCtx& q, GfShared& S, const GfTile& tl,
                                                const double (&X)[BR][BC], int K, const double* rr, double* dx) {
   const int t = q.t;
   const double* di = S.vec[V_DI];
-  double* w = S.vec[V_Z];
   if (t < q.n) dx[t] = di[t] * rr[t];
   __syncthreads();
   for (int k = 0; k < K; ++k) {
-    gf_tile_apply<BR, BC>(q, S, tl, X, dx, w);
-    if (t < q.n) dx[t] = fma(di[t], rr[t] - w[t], dx[t]);
+    // tile products of J dx into per-column-tile partial sums, then each row sums them and
+    // updates its dx entry: two barriers a step
+    if (tl.act) {
+#pragma unroll
+      for (int i = 0; i < BR; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < BC; ++j) acc = fma(X[i][j], dx[tl.tc * BC + j], acc);
+        S.red2[tl.tc * tl.Npad + tl.tr * BR + i] = acc;
+      }
+    }
+    __syncthreads();
+    if (t < q.n) {
+      double w = 0.0;
+      for (int c = 0; c < tl.TC; ++c) w += S.red2[c * tl.Npad + t];
+      dx[t] = fma(di[t], rr[t] - w, dx[t]);
+    }
     __syncthreads();
   }
 }
