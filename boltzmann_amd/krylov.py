@@ -124,15 +124,16 @@ class DeviceGMRES:
     is the least-squares residual |g_(i+1)| of the Givens-rotated Hessenberg system (the
     true residual in exact arithmetic, bz_gmres_givens), and the true residual is evaluated
     once per cycle, at its end, before the solution is accepted.  Per step: preconditioner,
-    operator, classical Gram-Schmidt twice (bz_multi_dot, bz_vt_axpy), the norm and scaling,
-    one rotation.  After the first cycle the steps are replayed from HIP graphs of `chunk`
+    operator, classical Gram-Schmidt (bz_multi_dot, bz_vt_axpy; two passes by default, one
+    with gs_passes=1), the norm and scaling, one rotation.  After the first cycle the steps are replayed from HIP graphs of `chunk`
     steps; the host reads the device stop flag between graphs.  The launchers' scalar
     arguments and buffers are captured with the graphs, so they must stay the same between
     solves (inputs are copied into fixed buffers)."""
 
-    def __init__(self, n, restart, dev, matvec, precond, chunk=4, use_graph=True):
+    def __init__(self, n, restart, dev, matvec, precond, chunk=4, use_graph=True, gs_passes=2):
         f64, i32 = torch.float64, torch.int32
         self.n, self.m, self.chunk = int(n), int(restart), int(chunk)
+        self.gs_passes = int(gs_passes)                             # classical Gram-Schmidt passes
         self.matvec, self.precond = matvec, precond
         self.use_graph = use_graph
         self.nblk = max(1, min(1024, (self.n + 4095) // 4096))
@@ -166,7 +167,7 @@ class DeviceGMRES:
         n, ldh = self.n, self.ldh
         self.precond(_at(self.V, i * n), _at(self.t))
         self.matvec(_at(self.t), _at(self.w))
-        for rep in range(2):
+        for rep in range(self.gs_passes):
             self._dot(self.V, i + 1, self.w, _at(self.H, i), ostride=ldh, accumulate=rep, out2=self.h)
             _lib.call("bz_vt_axpy", _at(self.V), n, i + 1, self.h, 1, -1.0, self.w, self.w, n)
         self._dot(self.w, 1, self.w, _at(self.H, (i + 1) * ldh + i), take_sqrt=1)
