@@ -489,6 +489,58 @@ int launch_fvm_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const do
   return BZ_OK;
 }
 
+// The x sweep for two-point stencils (px = 1) as an affine scan, one wavefront per (r, t)
+// system: along the sweep y_c = alpha_c + beta_c y_prev (alpha = b rden, beta = -av dt D_1 rden,
+// the inflow position has alpha = beta = 0); each lane composes the maps of PER consecutive
+// positions, a 6-step shuffle scan gives every lane the value entering its positions, and
+// the lane re-runs them.  The maps are contractions (upwind), so the re-association changes
+// the result at rounding level.
+template <int PER>
+__global__ __launch_bounds__(64) void k_fvm_x_scan(BzFvm1d3v op, double dt, const double* b, double* out) {
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const int Nvt = op.Nvt, Nx = op.Nx;
+  const int r = s / Nvt, t = s - r * Nvt;
+  const double av = op.axvr[r] * op.cos_vt[t];
+  const bool fwd = av >= 0;
+  const double* D = fwd ? op.x_l2r : op.x_r2l;
+  const double dtav = dt * av;
+  const long base = (long)s * Nx;
+  double al[PER], be[PER];
+  double A = 1.0, B = 0.0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int p = lane * PER + i;
+    al[i] = 0.0;
+    be[i] = 1.0;                                 // identity beyond the end
+    if (p < Nx) {
+      const int c = fwd ? p : Nx - 1 - p;
+      const double rden = 1.0 / (1.0 + dtav * D[c]);
+      al[i] = p == 0 ? 0.0 : b[base + c] * rden;
+      be[i] = p == 0 ? 0.0 : -dtav * D[Nx + (fwd ? c - 1 : c + 1)] * rden;
+    }
+    B = fma(be[i], B, al[i]);
+    A *= be[i];
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ap = __shfl_up(A, off, 64), Bp = __shfl_up(B, off, 64);
+    if (lane >= off) {
+      B = fma(A, Bp, B);
+      A *= Ap;
+    }
+  }
+  double y = __shfl_up(B, 1, 64);
+  if (lane == 0) y = 0.0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int p = lane * PER + i;
+    if (p < Nx) {
+      y = fma(be[i], y, al[i]);
+      out[base + (fwd ? p : Nx - 1 - p)] = y;
+    }
+  }
+}
+
 template <int MODE, int ND>
 void launch_sweep_nd(const BzFvm1d3v& op, int N, int nsys, const double* E, double dt, const double* b, double* out,
                      const double* h, hipStream_t st) {
@@ -590,6 +642,15 @@ int bz_fvm_precond_xvtvrc(const BzFvm1d3v* op, const double* E, double alpha_dt,
   if (rc != BZ_OK) return rc;
   rc = launch_fvm_sweep<SW_VT>(*op, E, alpha_dt, y, y, nullptr, stream);
   if (rc != BZ_OK) return rc;
+  if (op->nx == 2 && op->Nx <= 64 * 8) {
+    const dim3 grid(op->Nr * op->Nvt), block(64);
+    const int per = (op->Nx + 63) / 64;
+    if (per <= 2) hipLaunchKernelGGL((k_fvm_x_scan<2>), grid, block, 0, stream, *op, alpha_dt, y, y);
+    else if (per <= 4) hipLaunchKernelGGL((k_fvm_x_scan<4>), grid, block, 0, stream, *op, alpha_dt, y, y);
+    else hipLaunchKernelGGL((k_fvm_x_scan<8>), grid, block, 0, stream, *op, alpha_dt, y, y);
+    BZ_LAUNCH_CHECK();
+    return BZ_OK;
+  }
   return launch_fvm_sweep<SW_X>(*op, E, alpha_dt, y, y, nullptr, stream);
 }
 
