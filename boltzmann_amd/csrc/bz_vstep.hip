@@ -240,6 +240,38 @@ __global__ void k_gmres_back_solve(const double* __restrict__ H, int ldh, const 
 
 inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
+
+// C = A B, all row-major and contiguous: A (m, k), B (k, n), C (m, n), on the FP64 matrix
+// cores, one wavefront per 16 x 16 tile of C (so a product with a few hundred rows and
+// thousands of columns fills the chip), v_mfma_f64_16x16x4f64 over k in steps of 4.  Operand
+// layout of the instruction (measured on gfx950): A lane = i + 16 kk, B lane = j + 16 kk;
+// element q of the accumulator of lane l is D[4 q + l / 16][l % 16].  Rows, columns and k
+// beyond the ends are clamped on load (k padded with zero A entries) and not stored.
+typedef double bz_gd4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_dgemm_nn(const double* __restrict__ A, int m, int k,
+                                                 const double* __restrict__ B, int n, double* __restrict__ C) {
+  const int lane = threadIdx.x;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
+  const int row = min(r0 + i16, m - 1), col = min(c0 + i16, n - 1);
+  const double* Ar = A + (size_t)row * k;
+  bz_gd4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k0 = 0; k0 < k; k0 += 4) {
+    const int kk = k0 + k4;
+    const int kc = min(kk, k - 1);
+    const double a = kk < k ? Ar[kc] : 0.0;
+    const double bv = B[(size_t)kc * n + col];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int rr = r0 + 4 * q + k4, cc = c0 + i16;
+    if (rr < m && cc < n) C[(size_t)rr * n + cc] = acc[q];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -289,6 +321,15 @@ int bz_gmres_back_solve(const double* H, int ldh, const double* g, const int* kd
                         hipStream_t stream) {
   if (m < 1 || ldh < m || !H || !g || !kdone || !y) return BZ_EINVAL;
   hipLaunchKernelGGL(k_gmres_back_solve, dim3(1), dim3(64), 0, stream, H, ldh, g, kdone, y, m);
+  BZ_LAUNCH_CHECK();
+  return BZ_OK;
+}
+
+
+int bz_dgemm_nn(const double* A, int m, int k, const double* B, int n, double* C, hipStream_t stream) {
+  if (m < 1 || k < 1 || n < 1 || !A || !B || !C || C == A || C == B) return BZ_EINVAL;
+  if ((m + 15) / 16 > 65535) return BZ_EINVAL;
+  hipLaunchKernelGGL(k_dgemm_nn, dim3((n + 15) / 16, (m + 15) / 16), dim3(64), 0, stream, A, m, k, B, n, C);
   BZ_LAUNCH_CHECK();
   return BZ_OK;
 }

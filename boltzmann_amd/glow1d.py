@@ -129,10 +129,13 @@ class XAdvection:
         if dt is not None and dt != self.dt:
             raise AssertionError("x-advection operators were set up for another dt")
         Nr, Nvt, Nx = self.Nr, self.Nvt, self.Nx
-        z = (self.Qinv @ v.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx).contiguous()
+        z = torch.empty((Nr * Nvt, Nx), dtype=torch.float64, device=v.device)
+        _lib.call("bz_dgemm_nn", self.Qinv, Nr, Nr, v.contiguous(), Nvt * Nx, z)
         y = torch.empty_like(z)
         _lib.call("bz_xstep_bidiag_solve", self.diag, self.sdiag, self.lower_of, z, y, Nr, Nvt, Nx)
-        return (self.Q @ y.reshape(Nr, Nvt * Nx)).reshape(Nr * Nvt, Nx)
+        out = torch.empty((Nr * Nvt, Nx), dtype=torch.float64, device=y.device)
+        _lib.call("bz_dgemm_nn", self.Q, Nr, Nr, y.contiguous(), Nvt * Nx, out)
+        return out
 
 
 class _DeviceGMRES:

@@ -305,6 +305,11 @@ int bz_adv_x_be(const double* D_LtoR, const double* D_RtoL, const double* vr, co
 int bz_bsr_upper_unit_solve(const double* data, const int* indptr, const int* indices, int nbr, int nrhs, int bsz,
                             const double* b, double* out, hipStream_t stream);
 
+/* C = A B for row-major contiguous A (m, k), B (k, n), C (m, n) on the FP64 matrix cores.
+ * Replaces the dense eigenbasis products of step_bte_x, Qinv v and Q y with v (num_p, Nvt Nx)
+ * (glowdischarge_boltzmann_1d.py:1220-1287). */
+int bz_dgemm_nn(const double* A, int m, int k, const double* B, int n, double* C, hipStream_t stream);
+
 /* ------------------------------------------------------------------ GMRES building blocks */
 
 /* out[j*ostride] (+)= sum_e V[j*ldv + e] u[e] for j < nv (or its square root), two-pass

@@ -136,3 +136,21 @@ def test_glow1d_vspace_dropin_methods():
         w = xa.step(vs.step(xa.step(w, 0.5 * dt).contiguous(), E), 0.5 * dt)
         err = float((v - w).abs().max() / w.abs().max())
         assert err < 1e-12, (k, err)
+
+
+@pytest.mark.parametrize("m,k,n", [(129, 129, 6400), (17, 5, 33), (1, 1, 1), (300, 77, 50), (16, 4, 16)])
+def test_dgemm_nn_matches_fp64_product(m, k, n):
+    """bz_dgemm_nn (FP64 MFMA tiles, clamped edges) against the FP64 product on the host."""
+    import numpy as np
+    import torch
+    from boltzmann_amd import _lib
+    print(f"case: dgemm m={m} k={k} n={n}", flush=True)
+    rng = np.random.default_rng(m * 1000 + k * 10 + n)
+    A = rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n))
+    C = torch.empty((m, n), dtype=torch.float64, device="cuda")
+    _lib.call("bz_dgemm_nn", torch.from_numpy(A).cuda(), m, k, torch.from_numpy(B).cuda(), n, C)
+    want = A @ B
+    scale = np.abs(A) @ np.abs(B)
+    err = np.abs(C.cpu().numpy() - want) / np.maximum(scale, 1e-300)
+    assert err.max() < 4 * k * 2.2e-16, err.max()
