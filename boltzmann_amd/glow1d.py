@@ -149,16 +149,19 @@ class _DeviceGMRES:
     the true residual is evaluated once per cycle, at its end.  The cycle is replayed from
     HIP graphs of `chunk` steps each (captured on first use); the host reads the device stop
     flag between chunks.  Per step: the binned preconditioner (bz_binned_apply), the operator
-    product (bz_vstep_matvec), classical Gram-Schmidt twice (bz_multi_dot, bz_vt_axpy), the
-    norm, the scaling and one Givens rotation (bz_gmres_givens): 9 launches, none of them
-    reading back to the host."""
+    product (bz_vstep_matvec), classical Gram-Schmidt (bz_multi_dot, bz_vt_axpy; one pass,
+    gs_passes=2 for two), the norm, the scaling and one Givens rotation (bz_gmres_givens):
+    7 launches (9 with two passes), none of them reading back to the host."""
 
-    def __init__(self, vs, nx, chunk=4):
+    def __init__(self, vs, nx, chunk=4, gs_passes=1):
         dev = vs.dev
+        # classical Gram-Schmidt passes per step: one (the cost of the reference's single
+        # modified pass); the v-steps take the same number of steps as with two
         self.vs, self.nx, self.dof = vs, nx, vs.dof
         self.n = n = vs.dof * nx
         self.m = m = vs.rsrt
         self.chunk = chunk
+        self.gs_passes = int(gs_passes)
         f64, i32 = torch.float64, torch.int32
         self.nblk = max(1, min(1024, (n + 4095) // 4096))
         self.V = torch.empty((m + 1, n), dtype=f64, device=dev)
@@ -206,7 +209,7 @@ class _DeviceGMRES:
         done = self._done()
         self._precond(_ptr(self.V, i * n), self.t, done)
         self._matvec(self.t, self.w, done)
-        for rep in range(2):
+        for rep in range(self.gs_passes):
             self._dot(self.V, i + 1, self.w, _ptr(self.H, i), ostride=ldh, accumulate=rep, out2=self.h)
             _lib.call("bz_vt_axpy", _ptr(self.V), n, i + 1, self.h, 1, -1.0, self.w, self.w, n)
         self._dot(self.w, 1, self.w, _ptr(self.H, (i + 1) * ldh + i), take_sqrt=1)
