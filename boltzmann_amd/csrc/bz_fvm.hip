@@ -187,20 +187,23 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
   y[e] = out;
 }
 
-// h = H s on the matrix cores: one workgroup of four wavefronts per 16 x 16 tile of h (rows
-// r, points x).  The columns r' run from the first nonzero column of the tile's rows (H is
-// upper triangular: collisions only move electrons down in energy) to Nr in steps of 4, split
-// in four ranges, one per wavefront, each accumulated by v_mfma_f64_16x16x4f64; the four
+// h = H s on the matrix cores: one workgroup of CC_WAVES wavefronts per 16 x 16 tile of h
+// (rows r, points x).  The columns r' run from the first nonzero column of the tile's rows (H
+// is upper triangular: collisions only move electrons down in energy) to Nr in steps of 4,
+// split in CC_WAVES ranges, one per wavefront, each accumulated by v_mfma_f64_16x16x4f64; the
 // partial tiles are summed through LDS.  Operand layout of the instruction (measured on
 // gfx950): A (16 x 4) lane = i + 16 k, B (4 x 16) lane = j + 16 k; element m of the
 // accumulator of lane l is D[4 m + l / 16][l % 16].  Rows and points beyond the ends are
 // clamped on load and not stored.
 typedef double bz_d4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_fvm_coll_corr(const double* __restrict__ H, const int* __restrict__ h_lo,
-                                                       const double* __restrict__ s, int Nr, int Nx,
-                                                       double* __restrict__ h) {
-  __shared__ double red[3][256];
+constexpr int CC_WAVES = 8;
+
+__global__ __launch_bounds__(64 * CC_WAVES) void k_fvm_coll_corr(const double* __restrict__ H,
+                                                                 const int* __restrict__ h_lo,
+                                                                 const double* __restrict__ s, int Nr, int Nx,
+                                                                 double* __restrict__ h) {
+  __shared__ double red[CC_WAVES - 1][256];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i16 = lane & 15, k4 = lane >> 4;
   const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(256) void k_fvm_coll_corr(const double* __restrict_
   kmin = __shfl(kmin, 0, 64);
   const int k0 = kmin & ~3;
   const int nsteps = (Nr - k0 + 3) >> 2;
-  const int per = (nsteps + 3) >> 2;
+  const int per = (nsteps + CC_WAVES - 1) / CC_WAVES;
   const int s_lo = wave * per, s_hi = min(nsteps, s_lo + per);
   const double* Hr = H + (size_t)row * Nr;
   bz_d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -231,7 +234,9 @@ __global__ __launch_bounds__(256) void k_fvm_coll_corr(const double* __restrict_
   if (wave == 0) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double v = acc[m] + red[0][m * 64 + lane] + red[1][m * 64 + lane] + red[2][m * 64 + lane];
+      double v = acc[m];
+#pragma unroll
+      for (int w = 0; w < CC_WAVES - 1; ++w) v += red[w][m * 64 + lane];
       const int rr = r0 + 4 * m + k4, cc = c0 + i16;
       if (rr < Nr && cc < Nx) h[(size_t)rr * Nx + cc] = v;
     }
@@ -630,8 +635,8 @@ int bz_fvm_precond_xvtvrc(const BzFvm1d3v* op, const double* E, double alpha_dt,
   hipLaunchKernelGGL(k_fvm_moment, dim3((nrx + 255) / 256), dim3(256), 0, stream, b, op->wq, op->Nr, op->Nvt, op->Nx,
                      s);
   BZ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((op->Nr + 15) / 16, (op->Nx + 15) / 16), dim3(256), 0, stream, op->H,
-                     op->h_lo, s, op->Nr, op->Nx, h);
+  hipLaunchKernelGGL(k_fvm_coll_corr, dim3((op->Nr + 15) / 16, (op->Nx + 15) / 16), dim3(64 * CC_WAVES), 0, stream,
+                     op->H, op->h_lo, s, op->Nr, op->Nx, h);
   BZ_LAUNCH_CHECK();
   // the radial sweep: segmented scan for the common stencils and long radial lines, else one
   // thread per line
