@@ -218,13 +218,15 @@ class _DeviceGMRES:
                   _ptr(self.flags, 1))
 
     def _capture(self):
-        self.graphs = []
-        for i0 in range(0, self.m, self.chunk):
-            gr = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(gr):
-                for i in range(i0, min(self.m, i0 + self.chunk)):
-                    self._step(i)
-            self.graphs.append(gr)
+        from .krylov import capture_guard
+        with capture_guard():
+            self.graphs = []
+            for i0 in range(0, self.m, self.chunk):
+                gr = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gr):
+                    for i in range(i0, min(self.m, i0 + self.chunk)):
+                        self._step(i)
+                self.graphs.append(gr)
 
     def _cycle_start(self, atol, rtol):
         """r = b - L P x, |r| -> scal[0], V[0] = r / |r|, H = 0, g, tol and the flags set."""

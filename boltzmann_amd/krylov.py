@@ -16,6 +16,7 @@ Givens rotations.  The host reads one scalar per step (the true residual norm), 
 the reference does.
 """
 import ctypes
+import gc
 
 import torch
 
@@ -113,6 +114,26 @@ def _ptr(t, offset=0):
     return ctypes.c_void_p(t.data_ptr() + t.element_size() * offset)
 
 
+class capture_guard:
+    """Collects dead Python reference cycles before a stream capture and keeps the cyclic
+    garbage collector off during it.  A dead cycle can hold captured graphs or device memory
+    (a solver and its GMRES object referring to each other); if the collector frees them in
+    the middle of another capture, the graph's teardown and the freeing of its memory pool
+    are device calls that a capturing stream does not allow, and the process aborts.  This
+    torch release no longer collects at the start of a capture by default."""
+
+    def __enter__(self):
+        gc.collect()
+        self._was = gc.isenabled()
+        gc.disable()
+        return self
+
+    def __exit__(self, *exc):
+        if self._was:
+            gc.enable()
+        return False
+
+
 class DeviceGMRES:
     """Restarted, right-preconditioned GMRES with the whole Arnoldi step on the device, for an
     operator and a preconditioner given as launchers `matvec(src, dst)` and
@@ -176,13 +197,14 @@ class DeviceGMRES:
                   _ptr(self.flags, 1))
 
     def _capture(self):
-        self.graphs = []
-        for i0 in range(0, self.m, self.chunk):
-            gr = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(gr):
-                for i in range(i0, min(self.m, i0 + self.chunk)):
-                    self._step(i)
-            self.graphs.append(gr)
+        with capture_guard():
+            self.graphs = []
+            for i0 in range(0, self.m, self.chunk):
+                gr = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gr):
+                    for i in range(i0, min(self.m, i0 + self.chunk)):
+                        self._step(i)
+                self.graphs.append(gr)
 
     def _residual(self, out_scal):
         """r = b - A P x, |r| into scal[out_scal]; leaves P x in t."""
