@@ -221,13 +221,16 @@ class DeviceGMRES:
         else:
             self.x.copy_(x0.reshape(-1))
         self._dot(self.b, 1, self.b, _at(self.scal, 1), take_sqrt=1)
+        self._residual(0)                                  # r = b - A P x0, |r| -> scal[0]
         iters = 0
         for cyc in range(max_cycles):
-            self._residual(0)
+            # r and |r| are those of the current x: computed above, or at the end of the
+            # previous cycle (where the host found |r| >= tol), so only the first cycle reads
+            # the start test back
             self.H.zero_()
             _lib.call("bz_gmres_cycle_start", self.scal, float(atol), float(rtol), self.m, self.g, self.tol,
                       _ptr(self.flags, 0), _ptr(self.flags, 1))
-            if int(self.flags[0].item()):
+            if cyc == 0 and int(self.flags[0].item()):
                 return self.t.clone(), iters
             _lib.call("bz_scale_inv", self.r, self.scal, _at(self.V, 0), self.n)
             if self.graphs is None or not self.use_graph:
@@ -243,14 +246,16 @@ class DeviceGMRES:
                     gr.replay()
                     if int(self.flags[0].item()):
                         break
-            k = int(self.flags[1].item())
-            iters += k
             _lib.call("bz_gmres_back_solve", self.H, self.ldh, self.g, _ptr(self.flags, 1), self.y, self.m)
             _lib.call("bz_vt_axpy", _at(self.V), self.n, self.m, self.y, 1, 1.0, self.x, self.x, self.n)
             self._residual(2)
-            rn, bn, tol = (float(v) for v in torch.stack([self.scal[2], self.scal[1], self.tol[0]]).tolist())
+            # one read-back per cycle: the true residual, |b|, the tolerance and the step count
+            rn, bn, tol, k = torch.stack([self.scal[2], self.scal[1], self.tol[0],
+                                          self.flags[1].to(torch.float64)]).tolist()
+            iters += int(k)
             if callback is not None:
                 callback(rn / bn if bn > 0 else 0.0)
             if rn < tol:
                 return self.t.clone(), iters
+            self.scal[0].copy_(self.scal[2])              # the next cycle starts from this r
         raise RuntimeError(f"GMRES did not converge in {max_cycles} cycles of {self.m}")
