@@ -265,16 +265,16 @@ class _DeviceGMRES:
                     gr.replay()
                     if int(self.flags[0].item()):
                         break
-            k = int(self.flags[1].item())
-            iters += k
             _lib.call("bz_gmres_back_solve", self.H, self.ldh, self.g, _ptr(self.flags, 1), self.y, self.m)
             _lib.call("bz_vt_axpy", _ptr(self.V), self.n, self.m, self.y, 1, 1.0, self.x, self.x, self.n)
-            # the true residual of the cycle's end
+            # the true residual of the cycle's end; one read-back for it, the tolerance and the
+            # cycle's step count
             self._precond(self.x, self.t)
             self._matvec(self.t, self.r)
             torch.sub(self.b, self.r, out=self.r)
             self._dot(self.r, 1, self.r, _ptr(self.scal, 2), take_sqrt=1)
-            rn, tol = float(self.scal[2].item()), float(self.tol[0].item())
+            rn, tol, k = torch.stack([self.scal[2], self.tol[0], self.flags[1].to(torch.float64)]).tolist()
+            iters += int(k)
             if rn < tol:
                 return self.t.reshape(self.dof, self.nx).clone(), iters
         raise RuntimeError(f"v-step GMRES did not converge in {max_cycles} cycles of {self.m}")
