@@ -28,8 +28,17 @@ __global__ __launch_bounds__(256) void k_fvm_moment(const double* __restrict__ v
   if (e >= Nr * Nx) return;
   const int r = e / Nx, ix = e - r * Nx;
   const double* p = v + (size_t)r * Nvt * Nx + ix;
+  // eight ordinates' loads in flight at a time (the sum keeps the ordinate order)
   double acc = 0.0;
-  for (int t = 0; t < Nvt; ++t) acc += wq[t] * p[(size_t)t * Nx];
+  int t = 0;
+  for (; t + 8 <= Nvt; t += 8) {
+    double q[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) q[j] = p[(size_t)(t + j) * Nx];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc += wq[t + j] * q[j];
+  }
+  for (; t < Nvt; ++t) acc += wq[t] * p[(size_t)t * Nx];
   s[e] = acc;
 }
 
