@@ -126,11 +126,12 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
   const bool low = t >= mp;
   const double cx = op.dt_x * op.adx[r] * cs;
   const double* DX = low ? op.dxL : op.dxR;
+  const double x0 = x[e];
   double xv[NX], dx[NX];
 #pragma unroll
   for (int k = 0; k < NX; ++k) {
     const int j = min(max(low ? ix - k : ix + k, 0), Nx - 1);
-    xv[k] = x[e + (j - ix)];
+    xv[k] = k == 0 ? x0 : x[e + (j - ix)];
     dx[k] = DX[k * Nx + ix];
   }
   double yx = 0.0;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
 #pragma unroll
     for (int k = 0; k < GMAX; ++k) g += (q0 + k < q1) ? gv[k] * gs[k] : 0.0;
     for (int q = q0 + GMAX; q < q1; ++q) g += op.g_val[q] * s[(size_t)op.g_col[q] * Nx + ix];
-    const double xc = x[e];
+    const double xc = x0;
     const double cv = g - op.loss[r] * xc;
     const double Ex = E[ix];
     const bool ep = Ex >= 0;
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
 #pragma unroll
     for (int d = 0; d < NVR; ++d) {
       const int j = min(max(vr_up ? r + d : r - d, 0), Nr - 1);
-      xr[d] = x[e + (long)(j - r) * rs];
+      xr[d] = d == 0 ? x0 : x[e + (long)(j - r) * rs];
       dr[d] = Dr[d * Nr + r];
     }
     double ar = 0.0;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_fvm_matvec_w(BzFvm1d3v op, const double
 #pragma unroll
     for (int d = 0; d < NVT; ++d) {
       const int j = min(max(ep ? t + d : t - d, 0), Nvt - 1);
-      xt[d] = x[e + (long)(j - t) * Nx];
+      xt[d] = d == 0 ? x0 : x[e + (long)(j - t) * Nx];
       dtt[d] = Dt[d * Nvt + t];
     }
     double at = 0.0;
@@ -620,9 +621,9 @@ int bz_fvm_rhs_mvec(const BzFvm1d3v* op, const double* E, double dt, const doubl
   const dim3 growx((unsigned)((nwg + 7) / 8 * 8)), browx(tx);
   const bool rows_fit = nwg < (1L << 30);
   if (rows_fit && op->nx == 2 && op->nvr == 4 && op->nvt == 4)
-    hipLaunchKernelGGL((k_fvm_matvec_w<2, 4, 4, 8>), growx, browx, 0, stream, *op, E, dt, x, s, y);
+    hipLaunchKernelGGL((k_fvm_matvec_w<2, 4, 4, 6>), growx, browx, 0, stream, *op, E, dt, x, s, y);
   else if (rows_fit && op->nx == 3 && op->nvr == 3 && op->nvt == 3)
-    hipLaunchKernelGGL((k_fvm_matvec_w<3, 3, 3, 8>), growx, browx, 0, stream, *op, E, dt, x, s, y);
+    hipLaunchKernelGGL((k_fvm_matvec_w<3, 3, 3, 6>), growx, browx, 0, stream, *op, E, dt, x, s, y);
   else
     hipLaunchKernelGGL(k_fvm_matvec, grid, block, 0, stream, *op, E, dt, x, s, y);
   BZ_LAUNCH_CHECK();
