@@ -361,9 +361,12 @@ class bte_1d3v:
         self._t.update(g_ptr=d(np.array(gp), torch.int32), g_col=d(np.array(gc if gc else [0]), torch.int32),
                        g_val=d(np.array(gv if gv else [0.0])))
         self._work = torch.empty(2 * Nr * Nx, dtype=torch.float64, device=self.dev)
-        self.op_mass_d = d(self.op_mass)
-        self.op_temp_d = d(self.op_temp)
-        self.op_rate_d = d(np.stack(self.op_rate))
+        # the QoI weight vectors [mass, temp, rate_0, ...] in groups of three (zero padded) for
+        # the one-pass weighted sums of bz_glow_moments
+        wts = [self.op_mass, self.op_temp] + list(self.op_rate)
+        wts += [np.zeros(self.dof_v)] * (-len(wts) % 3)
+        self._qoi_w = [d(np.stack(wts[i:i + 3])) for i in range(0, len(wts), 3)]
+        self._qoi_split = 64
 
     # -------------------------------------------------------------- state and QoIs
     def initial_condition(self, type=0):
@@ -379,15 +382,28 @@ class bte_1d3v:
         v = np.repeat(hv, self.dof_vt)[:, None] * ne[None, :]
         return torch.from_numpy(np.ascontiguousarray(v)).to(self.dev)
 
+    def _weighted_sums(self, v):
+        """(2 + processes, Nx) sums over the state rows of [mass, temp, rate_k] x v: one pass of
+        bz_glow_moments per three weight vectors (64 row splits), then the split sums."""
+        v = v.reshape(self.dof_v, self.dof_x).contiguous()
+        out = []
+        for W in self._qoi_w:
+            part = torch.empty((self._qoi_split, 3, self.dof_x), dtype=torch.float64, device=self.dev)
+            _lib.call("bz_glow_moments", v, self.dof_v, self.dof_x, W, self._qoi_split, part)
+            out.append(part.sum(0))
+        return torch.cat(out)[:2 + self.num_collisions]
+
     def normalize_edf(self, v):
-        return v / ((self.op_mass_d @ v) * (2 * (self.bs_vth / C_GAMMA) ** 3))
+        m0 = self._weighted_sums(v)[0]
+        return v / (m0 * (2 * (self.bs_vth / C_GAMMA) ** 3))
 
     def compute_qoi(self, v, time=0.0, dt=0.0):
-        """bte_1d3v_solver.py:1179-1216 (FVM): ne, Te and the rate coefficients per x point."""
-        m0 = self.op_mass_d @ v
-        Te = (self.op_temp_d @ v) / m0
-        vn = self.normalize_edf(v)
-        rates = (self.op_rate_d @ vn).T
+        """bte_1d3v_solver.py:1179-1216 (FVM): ne, Te and the rate coefficients per x point.
+        The rates of the normalised EDF are the rate sums of v over m0 2 (vth / c_gamma)^3."""
+        S = self._weighted_sums(v)
+        m0 = S[0]
+        Te = S[1] / m0
+        rates = (S[2:] / (m0 * (2 * (self.bs_vth / C_GAMMA) ** 3))).T
         return {"rates": rates, "mu": 0, "D": 0, "ne": m0 * self.params.np0, "Te": Te}
 
     # -------------------------------------------------------------- init

@@ -131,6 +131,7 @@ def test_time_steps_and_moments():
     for key in ("ne", "Te"):
         assert _rel(qd[key], q[key]) < 1e-8, key
     assert _rel(qd["rates"], q["rates"]) < 1e-8
+    assert _rel(S.normalize_edf(vd), O.normalize_edf(v)) < 1e-8
 
 
 @pytest.mark.parametrize("device_gmres", [True, False])
