@@ -382,11 +382,16 @@ __global__ __launch_bounds__(SWT) void k_fvm_sweep(BzFvm1d3v op, const double* _
 // (consecutive x) times NSEG segments, so loads of one segment are coalesced over x.
 constexpr int SB = 8, NSEG = 32, SCAN_T = SB * NSEG;
 
-template <int ND>
+// MAXL > 0: a segment has at most MAXL positions, and all of its loads (right-hand sides and
+// stencil weights, from clamped in-bounds positions) are issued up front into registers and kept
+// for pass 2, so each pass waits on memory once instead of once per position; the arithmetic,
+// and its order, are those of the MAXL = 0 form, which loads one position at a time.
+template <int ND, int MAXL>
 __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const double* __restrict__ E, double dt,
                                                         const double* __restrict__ b, double* __restrict__ out,
                                                         const double* __restrict__ h) {
   constexpr int K = ND > 1 ? ND - 1 : 1;
+  constexpr int NL = MAXL > 0 ? MAXL : 1;
   __shared__ double maps[SCAN_T][K * K + K];
   const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
   const int N = Nr;
@@ -409,6 +414,11 @@ __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const doub
 #pragma unroll
     for (int d = 0; d < ND; ++d) dd[d] = D[d * N + (fwd ? max(c - d, 0) : min(c + d, N - 1))];
   };
+  double bv[NL], dv[NL][ND];
+  if constexpr (MAXL > 0) {
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i) coeffs(min(p_lo + i, N - 1), bv[i], dv[i]);
+  }
   // pass 1: the segment's map, state (y0 .. y_(K-1)), y0 the newest value
   double A[K][K], cv[K];
 #pragma unroll
@@ -417,30 +427,39 @@ __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const doub
 #pragma unroll
     for (int q = 0; q < K; ++q) A[r][q] = (r == q) ? 1.0 : 0.0;
   }
+  auto compose = [&](double bi, const double dd[ND]) __attribute__((always_inline)) {
+    const double rden = 1.0 / (1.0 + av * dt * dd[0]);
+    double a0[K], c0 = bi * rden;
+#pragma unroll
+    for (int q = 0; q < K; ++q) a0[q] = 0.0;
+#pragma unroll
+    for (int d = ND - 1; d >= 1; --d) {
+      const double beta = dtav * dd[d] * rden;
+      c0 -= beta * cv[d - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) a0[q] -= beta * A[d - 1][q];
+    }
+#pragma unroll
+    for (int r = K - 1; r >= 1; --r) {
+      cv[r] = cv[r - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) A[r][q] = A[r - 1][q];
+    }
+    cv[0] = c0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) A[0][q] = a0[q];
+  };
   if (active) {
-    for (int p = p_lo; p < p_hi; ++p) {
-      double bi, dd[ND];
-      coeffs(p, bi, dd);
-      const double rden = 1.0 / (1.0 + av * dt * dd[0]);
-      double a0[K], c0 = bi * rden;
+    if constexpr (MAXL > 0) {
 #pragma unroll
-      for (int q = 0; q < K; ++q) a0[q] = 0.0;
-#pragma unroll
-      for (int d = ND - 1; d >= 1; --d) {
-        const double beta = dtav * dd[d] * rden;
-        c0 -= beta * cv[d - 1];
-#pragma unroll
-        for (int q = 0; q < K; ++q) a0[q] -= beta * A[d - 1][q];
+      for (int i = 0; i < MAXL; ++i)
+        if (p_lo + i < p_hi) compose(bv[i], dv[i]);
+    } else {
+      for (int p = p_lo; p < p_hi; ++p) {
+        double bi, dd[ND];
+        coeffs(p, bi, dd);
+        compose(bi, dd);
       }
-#pragma unroll
-      for (int r = K - 1; r >= 1; --r) {
-        cv[r] = cv[r - 1];
-#pragma unroll
-        for (int q = 0; q < K; ++q) A[r][q] = A[r - 1][q];
-      }
-      cv[0] = c0;
-#pragma unroll
-      for (int q = 0; q < K; ++q) A[0][q] = a0[q];
     }
   }
 #pragma unroll
@@ -469,9 +488,7 @@ __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const doub
     for (int r = 0; r < K; ++r) y[r] = yn[r];
   }
   // pass 2: the sequential recurrence over the segment
-  for (int p = p_lo; p < p_hi; ++p) {
-    double bi, dd[ND];
-    coeffs(p, bi, dd);
+  auto advance = [&](int p, double bi, const double dd[ND]) __attribute__((always_inline)) {
     double sv = 0.0;
 #pragma unroll
     for (int d = ND - 1; d >= 1; --d) sv += dd[d] * y[d - 1];
@@ -481,6 +498,17 @@ __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const doub
     for (int d = K - 1; d >= 1; --d) y[d] = y[d - 1];
     y[0] = val;
     out[base + (fwd ? p : N - 1 - p) * stride] = val;
+  };
+  if constexpr (MAXL > 0) {
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i)
+      if (p_lo + i < p_hi) advance(p_lo + i, bv[i], dv[i]);
+  } else {
+    for (int p = p_lo; p < p_hi; ++p) {
+      double bi, dd[ND];
+      coeffs(p, bi, dd);
+      advance(p, bi, dd);
+    }
   }
 }
 
@@ -488,7 +516,11 @@ template <int ND>
 void launch_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out, const double* h,
                     hipStream_t st) {
   const int nsys = op.Nvt * op.Nx;
-  hipLaunchKernelGGL((k_fvm_vr_scan<ND>), dim3((nsys + SB - 1) / SB), dim3(SCAN_T), 0, st, op, E, dt, b, out, h);
+  const int L = (op.Nr + NSEG - 1) / NSEG;
+  const dim3 grid((nsys + SB - 1) / SB), block(SCAN_T);
+  if (L <= 8) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 8>), grid, block, 0, st, op, E, dt, b, out, h);
+  else if (L <= 24) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 24>), grid, block, 0, st, op, E, dt, b, out, h);
+  else hipLaunchKernelGGL((k_fvm_vr_scan<ND, 0>), grid, block, 0, st, op, E, dt, b, out, h);
 }
 
 int launch_fvm_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out,

@@ -79,7 +79,11 @@ def test_rhs_mvec_matches_oracle(kw):
     assert _rel(got, want) < 1e-13, _rel(got, want)
 
 
-@pytest.mark.parametrize("kw", CASES)
+# 319 radial points: segments of 10 points, the radial scan's 24-point register form
+LONG_RADIAL = [dict(Nr=31, spline_qpts=11, Np=6), dict(Nr=31, spline_qpts=11, Np=7, pvr=2, ev_extend=1)]
+
+
+@pytest.mark.parametrize("kw", CASES + LONG_RADIAL)
 def test_preconditioner_matches_oracle(kw):
     S, O = _pair(**kw)
     dt = 2e-3
