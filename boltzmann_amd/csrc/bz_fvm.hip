@@ -380,19 +380,19 @@ __global__ __launch_bounds__(SWT) void k_fvm_sweep(BzFvm1d3v op, const double* _
 // The maps are contractions (upwind, 1 + av dt D_0 >= 1 in the direction of the sweep), so
 // the re-association changes the result at rounding level.  A workgroup holds SB systems
 // (consecutive x) times NSEG segments, so loads of one segment are coalesced over x.
-constexpr int SB = 8, NSEG = 32, SCAN_T = SB * NSEG;
+constexpr int SCAN_SB = 8, NSEG = 32;
 
 // MAXL > 0: a segment has at most MAXL positions, and all of its loads (right-hand sides and
 // stencil weights, from clamped in-bounds positions) are issued up front into registers and kept
 // for pass 2, so each pass waits on memory once instead of once per position; the arithmetic,
 // and its order, are those of the MAXL = 0 form, which loads one position at a time.
-template <int ND, int MAXL>
-__global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const double* __restrict__ E, double dt,
+template <int ND, int MAXL, int SB>
+__global__ __launch_bounds__(SB * NSEG) void k_fvm_vr_scan(BzFvm1d3v op, const double* __restrict__ E, double dt,
                                                         const double* __restrict__ b, double* __restrict__ out,
                                                         const double* __restrict__ h) {
   constexpr int K = ND > 1 ? ND - 1 : 1;
   constexpr int NL = MAXL > 0 ? MAXL : 1;
-  __shared__ double maps[SCAN_T][K * K + K];
+  __shared__ double maps[SB * NSEG][K * K + K];
   const int Nr = op.Nr, Nvt = op.Nvt, Nx = op.Nx;
   const int N = Nr;
   const int j = threadIdx.x % SB, seg = threadIdx.x / SB;
@@ -512,24 +512,24 @@ __global__ __launch_bounds__(SCAN_T) void k_fvm_vr_scan(BzFvm1d3v op, const doub
   }
 }
 
-template <int ND>
+template <int ND, int SB>
 void launch_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out, const double* h,
                     hipStream_t st) {
   const int nsys = op.Nvt * op.Nx;
   const int L = (op.Nr + NSEG - 1) / NSEG;
-  const dim3 grid((nsys + SB - 1) / SB), block(SCAN_T);
-  if (L <= 8) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 8>), grid, block, 0, st, op, E, dt, b, out, h);
-  else if (L <= 24) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 24>), grid, block, 0, st, op, E, dt, b, out, h);
-  else hipLaunchKernelGGL((k_fvm_vr_scan<ND, 0>), grid, block, 0, st, op, E, dt, b, out, h);
+  const dim3 grid((nsys + SB - 1) / SB), block(SB * NSEG);
+  if (L <= 8) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 8, SB>), grid, block, 0, st, op, E, dt, b, out, h);
+  else if (L <= 24) hipLaunchKernelGGL((k_fvm_vr_scan<ND, 24, SB>), grid, block, 0, st, op, E, dt, b, out, h);
+  else hipLaunchKernelGGL((k_fvm_vr_scan<ND, 0, SB>), grid, block, 0, st, op, E, dt, b, out, h);
 }
 
 int launch_fvm_vr_scan(const BzFvm1d3v& op, const double* E, double dt, const double* b, double* out,
                        const double* h, hipStream_t st) {
   switch (op.nvr) {
-    case 2: launch_vr_scan<2>(op, E, dt, b, out, h, st); break;
-    case 3: launch_vr_scan<3>(op, E, dt, b, out, h, st); break;
-    case 4: launch_vr_scan<4>(op, E, dt, b, out, h, st); break;
-    case 5: launch_vr_scan<5>(op, E, dt, b, out, h, st); break;
+    case 2: launch_vr_scan<2, SCAN_SB>(op, E, dt, b, out, h, st); break;
+    case 3: launch_vr_scan<3, SCAN_SB>(op, E, dt, b, out, h, st); break;
+    case 4: launch_vr_scan<4, SCAN_SB>(op, E, dt, b, out, h, st); break;
+    case 5: launch_vr_scan<5, SCAN_SB>(op, E, dt, b, out, h, st); break;
     default: return BZ_EINVAL;
   }
   BZ_LAUNCH_CHECK();
