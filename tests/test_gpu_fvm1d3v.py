@@ -79,11 +79,14 @@ def test_rhs_mvec_matches_oracle(kw):
     assert _rel(got, want) < 1e-13, _rel(got, want)
 
 
-# 319 radial points: segments of 10 points, the radial scan's 24-point register form
-LONG_RADIAL = [dict(Nr=31, spline_qpts=11, Np=6), dict(Nr=31, spline_qpts=11, Np=7, pvr=2, ev_extend=1)]
+# 319 radial points: segments of 10 points, the radial scan's 24-point register form; then
+# the angular sweep with 2-, 3- and 5-point stencils and 16 ordinates, and 150 x points
+# (the x scan's four positions per lane)
+PC_CASES = [dict(Nr=31, spline_qpts=11, Np=6), dict(Nr=31, spline_qpts=11, Np=7, pvr=2, ev_extend=1),
+            dict(pvt=1, Nvt=16, Np=9), dict(pvt=2, Np=150), dict(pvt=4, Nvt=12, Np=10)]
 
 
-@pytest.mark.parametrize("kw", CASES + LONG_RADIAL)
+@pytest.mark.parametrize("kw", CASES + PC_CASES)
 def test_preconditioner_matches_oracle(kw):
     S, O = _pair(**kw)
     dt = 2e-3
